@@ -1,0 +1,320 @@
+/*
+ * pfchem.h -- C ABI of the MI355X per-cell chemistry step (operator-split
+ * reaction step of PFLOTRAN: RStep -> RReact -> RTotal / RActivityCoefficients
+ * / RKineticMineral / RImmobileDecay / SomDec sandbox / RSolve).
+ *
+ * Plain C, plain pointers and sizes.  Every entry point returns an int status
+ * (PFCHEM_OK == 0); pfchem_strerror() / pfchem_last_error() explain failures.
+ * No entry point aborts the process: the reference's `stop` /
+ * MPI_Abort paths become per-cell status codes the host acts on.
+ *
+ * Reference interfaces replaced (paths relative to the PFLOTRAN tree):
+ *   pfchem_step        <- RStep(guess, rt_auxvar, global_auxvar,
+ *                          material_auxvar, num_sub_steps, num_iterations,
+ *                          num_kinetic_state_updates, reaction, natural_id,
+ *                          option, ierror)   src/pflotran/reaction.F90:3564-3566,
+ *                        called once per cell from the OSRT loop
+ *                          src/pflotran/pmc_subsurface_osrt.F90:349-378;
+ *                        batched: one call per local cell range.
+ *   pfchem_update_activity <- RActivityCoefficients (reaction.F90:4368) as
+ *                        called per cell by RTUpdateActivityCoefficients
+ *                        (reactive_transport.F90:3441-3492) at the start of
+ *                        an OSRT step (pm_osrt.F90:148-151).
+ *   pfchem_residual    <- RTAccumulation + RReaction (reaction.F90:5710,
+ *                        4059): the per-cell residual of one RReact
+ *                        iterate, FP64 (used by GIRT callers and checks).
+ *   pfchem_tangent / pfchem_adjoint: no reference counterpart (the
+ *                        reference has no sensitivities); they differentiate
+ *                        the converged RReact system (reaction.F90:3853-4049).
+ *
+ * Layout: every per-cell field is structure-of-arrays, FP64, field-major:
+ * element (k, cell) of an array with per-cell length m lives at
+ * ptr[k * ld + cell], 0 <= k < m, 0 <= cell < ncell <= ld.
+ */
+#ifndef PFCHEM_H
+#define PFCHEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* return codes                                                        */
+/* ------------------------------------------------------------------ */
+enum {
+  PFCHEM_OK = 0,
+  PFCHEM_ERR_INVALID = 1,     /* bad argument / inconsistent network    */
+  PFCHEM_ERR_HIP = 2,         /* a HIP runtime call failed              */
+  PFCHEM_ERR_UNSUPPORTED = 3, /* feature/size not supported by a kernel */
+  PFCHEM_ERR_NOMEM = 4,       /* device allocation failed               */
+  PFCHEM_ERR_CELLS = 5        /* step ran; >= 1 cell reports failure    */
+};
+
+/* per-cell status written to pfchem_state.status (RStep ierror, refined) */
+enum {
+  PFCHEM_CELL_OK = 0,
+  PFCHEM_CELL_MAX_CUTS_ITERATIONS = 1, /* cuts exhausted, last RReact hit
+                                          maximum_reaction_iterations
+                                          (reaction.F90:3869-3895, 3662)   */
+  PFCHEM_CELL_MAX_CUTS_SINGULAR = 2,   /* cuts exhausted, last RReact had a
+                                          zero Jacobian row (utility.F90:629)*/
+  PFCHEM_CELL_MAX_CUTS_ACTIVITY = 3    /* cuts exhausted, last RReact had an
+                                          activity-coefficient failure
+                                          (reaction.F90:4421-4431,4477-4486) */
+};
+
+/* activity coefficient update frequency / algorithm
+   (same values as reaction_aux.F90:27-31) */
+enum {
+  PFCHEM_ACT_COEF_FREQUENCY_OFF = 0,
+  PFCHEM_ACT_COEF_FREQUENCY_TIMESTEP = 1,
+  PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER = 2,
+  PFCHEM_ACT_COEF_ALGORITHM_LAG = 3,
+  PFCHEM_ACT_COEF_ALGORITHM_NEWTON = 4
+};
+
+/* kernel variants for pfchem_step */
+enum {
+  PFCHEM_KERNEL_PLAIN_FP64 = 0,   /* one cell per thread, FP64 throughout   */
+  PFCHEM_KERNEL_BATCHED_FP64 = 1, /* persistent, work stealing, FP64 J/LU   */
+  PFCHEM_KERNEL_BATCHED_MIXED = 2 /* persistent, work stealing, FP32 J/LU,
+                                     FP64 residual / log K sums / update /
+                                     convergence test                       */
+};
+
+/* ------------------------------------------------------------------ */
+/* reaction network (host memory; copied into the context at create)   */
+/* mirrors reaction_rt_type fields (reaction_aux.F90:123-311) and       */
+/* mineral_type kinetic fields (reaction_mineral_aux.F90:82-140).       */
+/* All species ids are 0-based.                                        */
+/* ------------------------------------------------------------------ */
+typedef struct pfchem_somdec pfchem_somdec; /* SomDec sandbox, see below */
+
+typedef struct pfchem_network {
+  int32_t naqcomp;    /* reaction%naqcomp; aqueous dofs are 0..naqcomp-1   */
+  int32_t nimmobile;  /* reaction%immobile%nimmobile; dof naqcomp+i         */
+  int32_t h2o_aq_id;  /* species_idx%h2o_aq_id (0-based) or -1              */
+  int32_t pad0;
+  const double* primary_spec_Z;  /* [naqcomp]                               */
+  const double* primary_spec_a0; /* [naqcomp] (Angstrom)                    */
+
+  /* aqueous equilibrium complexes (reaction%eqcplx*) */
+  int32_t neqcplx;
+  int32_t eqcplx_maxsp;          /* row length of the two arrays below      */
+  const int32_t* eqcplxspecid;   /* [neqcplx][eqcplx_maxsp+1]; [k][0]=count  */
+  const double* eqcplxstoich;    /* [neqcplx][eqcplx_maxsp]                 */
+  const double* eqcplxh2ostoich; /* [neqcplx] or NULL (0 => no water)       */
+  const double* eqcplx_logK;     /* [neqcplx] at the reference temperature  */
+  const double* eqcplx_logKcoef; /* [neqcplx][5] or NULL (ReactionInterpolateLogK) */
+  const double* eqcplx_Z;        /* [neqcplx]                               */
+  const double* eqcplx_a0;       /* [neqcplx]                               */
+
+  /* kinetic minerals (reaction%mineral%kinmnrl*) */
+  int32_t nkinmnrl;
+  int32_t kinmnrl_maxsp;
+  const int32_t* kinmnrlspecid;      /* [nkinmnrl][kinmnrl_maxsp+1]         */
+  const double* kinmnrlstoich;       /* [nkinmnrl][kinmnrl_maxsp]           */
+  const double* kinmnrlh2ostoich;    /* [nkinmnrl] or NULL                  */
+  const double* kinmnrl_logK;        /* [nkinmnrl]                          */
+  const double* kinmnrl_logKcoef;    /* [nkinmnrl][5] or NULL               */
+  const double* kinmnrl_rate_constant;      /* [nkinmnrl] mol/m^2/s         */
+  const double* kinmnrl_activation_energy;  /* [nkinmnrl] J/mol, or NULL    */
+  const double* kinmnrl_molar_vol;          /* [nkinmnrl] m^3/mol           */
+  const double* kinmnrl_Temkin_const;       /* [nkinmnrl] or NULL (not associated) */
+  const double* kinmnrl_min_scale_factor;   /* [nkinmnrl] or NULL           */
+  const double* kinmnrl_affinity_power;     /* [nkinmnrl] or NULL           */
+  const double* kinmnrl_affinity_threshold; /* [nkinmnrl] or NULL (=0)      */
+  const double* kinmnrl_rate_limiter;       /* [nkinmnrl] or NULL (=0)      */
+  const int32_t* kinmnrl_irreversible;      /* [nkinmnrl] or NULL (=0)      */
+
+  /* first-order decay of immobile species (reaction%immobile%decay*) */
+  int32_t ndecay_rxn;
+  int32_t pad1;
+  const int32_t* decayspecid;          /* [ndecay_rxn] immobile index       */
+  const double* decay_rate_constant;   /* [ndecay_rxn] 1/s                  */
+
+  /* Debye-Hueckel constants (reaction_aux.F90:486-488 defaults) */
+  double debyeA, debyeB, debyeBdot;
+
+  /* optional SomDec decomposition sandbox (reaction_sandbox_somdec.F90) */
+  const pfchem_somdec* somdec;
+} pfchem_network;
+
+/* solver options: the reaction_rt_type control fields */
+typedef struct pfchem_options {
+  int32_t use_full_geochemistry;         /* reaction%use_full_geochemistry  */
+  int32_t use_log_formulation;           /* LOG_FORMULATION                 */
+  int32_t act_coef_update_frequency;     /* PFCHEM_ACT_COEF_FREQUENCY_*     */
+  int32_t act_coef_update_algorithm;     /* PFCHEM_ACT_COEF_ALGORITHM_*     */
+  int32_t use_activity_h2o;              /* ACTIVITY_H2O                    */
+  int32_t use_total_as_guess;            /* USE_TOTAL_CONCENTRATION_AS_GUESS*/
+  int32_t isothermal;                    /* option%use_isothermal           */
+  int32_t maximum_reaction_iterations;   /* default 20                      */
+  int32_t maximum_reaction_cuts;         /* default 10                      */
+  int32_t pad0;
+  double max_dlnC_rreact;                /* default 5                       */
+  double max_residual_tolerance;         /* default 1e-12                   */
+  double max_rel_residual_tolerance;     /* default 1e-8                    */
+  double max_relative_change_tolerance;  /* default 1e-6                    */
+} pfchem_options;
+
+/* ------------------------------------------------------------------ */
+/* SomDec sandbox descriptor (Setup output of SomDecSetup,             */
+/* reaction_sandbox_somdec.F90:987-1501), one entry per reaction.       */
+/* Species ids are 0-based: aqueous ids index pri/total, immobile ids   */
+/* index immobile.  -1 means "not present".                             */
+/* ------------------------------------------------------------------ */
+#define PFCHEM_SOMDEC_MAX_DOWN 4
+#define PFCHEM_SOMDEC_MAX_MONOD 4
+#define PFCHEM_SOMDEC_MAX_INHIB 4
+
+struct pfchem_somdec {
+  int32_t nrxn;
+  int32_t species_id_co2;        /* aqueous id (ITYPE_AQUEOUS)              */
+  int32_t species_id_nh4;        /* aqueous id                              */
+  int32_t species_id_no3;        /* aqueous id or -1                        */
+  int32_t species_id_n2o;        /* aqueous id or -1                        */
+  int32_t species_id_proton;     /* aqueous id or -1 (for N2O pH factor)    */
+  int32_t species_id_hr;         /* immobile id or -1 (tracking)            */
+  int32_t species_id_nmin;       /* immobile id or -1                       */
+  int32_t species_id_nimm;       /* immobile id or -1                       */
+  int32_t species_id_nimp;       /* immobile id or -1                       */
+  int32_t species_id_ngasmin;    /* immobile id or -1                       */
+  int32_t pad0;
+  double x0eps;                  /* this%x0eps                              */
+  double n2o_frac_mineralization;/* this%n2o_frac_mineralization            */
+  double inhibition_nh4_no3;     /* this%inhibition_nh4_no3                 */
+  /* per reaction arrays [nrxn] */
+  const double* rate_constant;       /* <0 => use rate_decomposition       */
+  const double* rate_decomposition;
+  const double* rate_ad_factor;
+  const int32_t* upstream_c_id;      /* immobile id                        */
+  const int32_t* upstream_n_id;      /* immobile id or -1                  */
+  const int32_t* upstream_hr_id;
+  const int32_t* upstream_nmin_id;
+  const int32_t* upstream_nimp_id;
+  const int32_t* upstream_nimm_id;
+  const double* upstream_nc;         /* setup-time N:C (mol/mol)           */
+  const int32_t* upstream_is_varycn;
+  const int32_t* n_downstream_pools;
+  const int32_t* downstream_c_id;    /* [nrxn][MAX_DOWN] immobile id       */
+  const int32_t* downstream_n_id;    /* [nrxn][MAX_DOWN]                   */
+  const double* downstream_stoich;   /* [nrxn][MAX_DOWN]                   */
+  const double* downstream_nc;       /* [nrxn][MAX_DOWN] setup-time N:C    */
+  const int32_t* downstream_is_varycn; /* [nrxn][MAX_DOWN]                 */
+  const double* mineral_c_stoich;    /* setup-time                         */
+  const double* mineral_n_stoich;    /* setup-time                         */
+  /* abiotic factors per reaction */
+  const int32_t* temperature_response_function; /* elm_rspfuncs ids, 0=off */
+  const double* Q10;
+  const double* Ea;
+  const int32_t* moisture_response_function;
+  const int32_t* Ox_response_function;
+  const double* Ox_half_saturation;
+  /* Monod / inhibition terms per reaction ([nrxn][MAX_*]) */
+  const int32_t* n_monod;
+  const int32_t* monod_specid;       /* aqueous id (>=0) or -(immobile id)-1 */
+  const double* monod_half_saturation;
+  const double* monod_threshold;
+  const int32_t* monod_pool_normalized;
+  const int32_t* n_inhibition;
+  const int32_t* inhibition_specid;
+  const int32_t* inhibition_type;
+  const double* inhibition_constant;
+  const double* inhibition_constant2;
+};
+
+/* ------------------------------------------------------------------ */
+/* per-cell state (SoA, see layout note at the top)                    */
+/* rt_auxvar / global_auxvar / material_auxvar fields used by RStep    */
+/* (reactive_transport_aux.F90:21-74, global_aux.F90, material_aux).   */
+/* ------------------------------------------------------------------ */
+typedef struct pfchem_state {
+  int64_t ld;                 /* stride between fields, >= ncell          */
+  double* pri_molal;          /* [naqcomp]  in: guess (free ion molality,
+                                 mol/kg); out: converged free molality     */
+  double* total;              /* [naqcomp]  in: total after transport
+                                 (mol/L); out: total at end of step        */
+  double* immobile;           /* [nimmobile] in/out (mol/m^3 bulk)         */
+  double* sec_molal;          /* [neqcplx]  in (LAG ionic strength) / out  */
+  double* pri_act_coef;       /* [naqcomp]  in/out                         */
+  double* sec_act_coef;       /* [neqcplx]  in/out                         */
+  double* ln_act_h2o;         /* [1] in/out, may be NULL if unused         */
+  double* mnrl_volfrac;       /* [nkinmnrl] in/out (m^3/m^3 bulk)          */
+  const double* mnrl_area;    /* [nkinmnrl] in (m^2/m^3 bulk)              */
+  double* mnrl_rate;          /* [nkinmnrl] out (mol/m^3 bulk/s)           */
+  const double* porosity;     /* [1]                                       */
+  const double* saturation;   /* [1] liquid saturation                     */
+  const double* den_kg;       /* [1] liquid density kg/m^3                 */
+  const double* volume;       /* [1] cell volume m^3                       */
+  const double* temperature;  /* [1] C (non-isothermal logK, Arrhenius)    */
+  const double* somdec_f_t;   /* [1] SomDec temperature scalar (ELM t_scalar)
+                                 or NULL => response functions            */
+  const double* somdec_f_w;   /* [1] SomDec moisture scalar (ELM w_scalar) */
+  int32_t* num_iterations;    /* [1] out: Newton iterations (RStep)        */
+  int32_t* num_sub_steps;     /* [1] out                                   */
+  int32_t* status;            /* [1] out: PFCHEM_CELL_*                    */
+} pfchem_state;
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct pfchem_ctx pfchem_ctx;
+
+/* Validate and copy the network to device `device`.  Replaces the
+   per-run setup of reaction_rt_type (ReactionCreate, reaction_aux.F90:371;
+   BasisInit, reaction_database.F90:812). */
+int pfchem_create(const pfchem_network* net, const pfchem_options* opt,
+                  int device, pfchem_ctx** out);
+int pfchem_destroy(pfchem_ctx* ctx);
+const char* pfchem_strerror(int code);
+/* message of the last failure on this context (never NULL) */
+const char* pfchem_last_error(const pfchem_ctx* ctx);
+
+/* One operator-split reaction step of length dt (option%tran_dt) for
+   ncell cells whose state pointers are DEVICE pointers.  Launches on
+   `stream` (a hipStream_t, NULL = default stream) and returns without
+   synchronizing.  Per-cell outcome goes to state->status; the call itself
+   returns PFCHEM_OK if the launch succeeded.
+   Replaces RStep (reaction.F90:3564) applied to every local cell. */
+int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                double dt, int variant, void* stream);
+
+/* Same as pfchem_step for HOST pointers: stages the state through device
+   buffers owned by the context, synchronizes, and returns PFCHEM_ERR_CELLS
+   if any cell failed (the reference's rstep_error all-reduce MAX,
+   pmc_subsurface_osrt.F90:381-382).  For a Fortran RStepBatch. */
+int pfchem_step_host(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                     double dt, int variant);
+
+/* RActivityCoefficients (reaction.F90:4368-4614) for every cell, device
+   pointers; uses the algorithm in the options. */
+int pfchem_update_activity(pfchem_ctx* ctx, int64_t ncell,
+                           const pfchem_state* state, void* stream);
+
+/* FP64 residual of the RReact system at the state's current free
+   molalities (reaction.F90:3899-3916): res[i*ld+cell] =
+   (Acc(x) - Acc(total^k,imm^k))/dt + reaction terms, with the fixed
+   accumulation taken from `fixed_total` / `fixed_immobile` (SoA).
+   Device pointers. */
+int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                    const double* fixed_total, const double* fixed_immobile,
+                    double dt, double* res, void* stream);
+
+/* Launch configuration chosen for the batched kernels, per size bucket;
+   set by the autotuner (lanes per cell, workgroup size, waves per CU). */
+typedef struct pfchem_launch {
+  int32_t lanes_per_cell;   /* 1 (cell per lane) or 8/16/32/64 (team)     */
+  int32_t block_size;       /* threads per workgroup (multiple of 64)     */
+  int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CUs      */
+  int32_t pad0;
+} pfchem_launch;
+int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg);
+int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFCHEM_H */

@@ -1,0 +1,332 @@
+"""Seeded synthetic workloads for the chemistry step.
+
+Networks and per-cell states are drawn from numpy's Philox bit generator,
+so the same (seed, sizes) gives bit-identical inputs on every host.
+
+calcite_workload (1000-cell grid, 3 primaries):
+    H+ / HCO3- / Ca++ with the five complexes of calcite.dat (OH-, CO3--,
+    CO2(aq), CaHCO3+, CaCO3(aq); stoichiometry and Z/a0 from
+    regression_tests/ascem/batch/calcite.dat:4-20) and one kinetic calcite
+    (-H+ + Ca++ + HCO3-, molar volume 36.934 cm^3/mol, calcite.dat:26),
+    k = 1e-9 mol/m^2/s, area 1 m^2/m^3, T = 25 C, dt = 3600 s, porosity 0.3,
+    saturation 1, V = 1 m^3, water density 997.05 kg/m^3.
+random_network_workload (15 primaries, 60 complexes, 5 minerals):
+    random networks with integer stoichiometry.
+
+Construction shared by both (stated here because the sizes alone do not
+fix it):
+  * log K values are drawn uniformly from the stated interval, and a draw is
+    kept only if the complex's molality at a reference composition lies in a
+    sane window (rejection sampling).  Without this filter random log K over
+    [-20, 15] with coefficients up to 3 gives complexes of 1e+30 mol/kg.
+  * each cell's previous free-ion molalities m0 are drawn; its previous
+    totals T0 follow from mass action with Debye-Hueckel activity
+    coefficients (LAG form, three fixed-point passes); the totals after
+    "transport" are T = (1-w) T0 + w T0[neighbour], w ~ U[0, wmax].  Totals
+    that are convex mixtures of attainable totals are attainable, so every
+    cell has a solution; the guess m0 is not it, so Newton has work to do.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from .network import CellState, ReactionNetwork, SolverOptions, DEBYE_A, DEBYE_B, DEBYE_BDOT
+from . import abi
+
+LOG_TO_LN = 2.30258509299  # pflotran_constants.F90:84
+WATER_DENSITY_25C = 997.05  # kg/m^3
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(key=int(seed)))
+
+
+@dataclass
+class Workload:
+    name: str
+    net: ReactionNetwork
+    opt: SolverOptions
+    state: CellState
+    dt: float
+    seed: int
+    notes: str = ""
+
+
+def bench_options(**kw) -> SolverOptions:
+    """Solver controls used for the cells/s metric: the reference defaults
+    except the convergence tests, tightened so that accepted iterates have
+    relative residual well below 1e-10 (the reference accepts at 1e-8 or on a
+    1e-6 relative change, reaction_aux.F90:533-537)."""
+    o = SolverOptions(
+        use_log_formulation=True,
+        act_coef_update_frequency=abi.ACT_COEF_FREQUENCY_TIMESTEP,
+        act_coef_update_algorithm=abi.ACT_COEF_ALGORITHM_LAG,
+        maximum_reaction_iterations=25,
+        maximum_reaction_cuts=10,
+        max_dlnC_rreact=5.0,
+        max_residual_tolerance=1.0e-30,
+        max_rel_residual_tolerance=1.0e-12,
+        max_relative_change_tolerance=1.0e-12,
+    )
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# ---------------------------------------------------------------------------
+# vectorized mass action (generator only; the oracle and kernels have their own)
+# ---------------------------------------------------------------------------
+def _lag_gamma(net: ReactionNetwork, m: np.ndarray, sec: np.ndarray):
+    """Debye-Hueckel LAG pass (same formula as reaction.F90:4555-4600)."""
+    Zp = net.primary_spec_Z[:, None]
+    Zc = net.eqcplx_Z[:, None]
+    I = 0.5 * ((m * Zp * Zp).sum(0) + (sec * Zc * Zc).sum(0))
+    sI = np.sqrt(I)
+
+    def g(Z, a0):
+        lg = -Z * Z * sI * net.debyeA / (1.0 + a0 * net.debyeB * sI) + net.debyeBdot * I
+        out = np.exp(lg * LOG_TO_LN)
+        return np.where(np.abs(Z) > 1e-10, out, 1.0)
+
+    return g(Zp, net.primary_spec_a0[:, None]), g(Zc, net.eqcplx_a0[:, None]), I
+
+
+def _speciate(net: ReactionNetwork, m: np.ndarray, gp: np.ndarray, gc: np.ndarray,
+              logK: Optional[np.ndarray] = None):
+    """sec molality and totals (mol/kg) at free molality m with fixed gammas."""
+    S = net.dense_stoichiometry()
+    lnact = np.log(m) + np.log(gp)
+    lk = net.eqcplx_logK if logK is None else logK
+    lnQK = -lk[:, None] * LOG_TO_LN + S @ lnact
+    if net.eqcplxh2ostoich is not None:
+        pass  # ln a_w = 0 (use_activity_h2o off)
+    sec = np.exp(lnQK) / gc
+    tot = m + S.T @ sec
+    return sec, tot
+
+
+def _equilibrate_gamma(net, m, passes=3):
+    sec = np.zeros((net.neqcplx, m.shape[1]))
+    gp = np.ones_like(m)
+    gc = np.ones_like(sec)
+    for _ in range(passes):
+        sec, _ = _speciate(net, m, gp, gc)
+        gp, gc, I = _lag_gamma(net, m, sec)
+    sec, tot = _speciate(net, m, gp, gc)
+    return sec, tot, gp, gc
+
+
+def _draw_positive_totals(net, draw, ncell, rounds=200):
+    """Draw free molalities (draw(k) -> (naq, k)) until every total is
+    positive.  RStep replaces totals <= 1e-40 by 1e-40 (reaction.F90:3617-3631),
+    which would turn a negative total into a different problem."""
+    m0 = draw(ncell)
+    for _ in range(rounds):
+        _, T0, _, _ = _equilibrate_gamma(net, m0)
+        bad = np.where((T0 <= 0.0).any(axis=0))[0]
+        if bad.size == 0:
+            return m0
+        m0[:, bad] = draw(bad.size)
+    raise RuntimeError("could not draw cells with positive totals")
+
+
+def _fill_common(st: CellState, net: ReactionNetwork, m0, sec0, gp, gc, T_molal,
+                 den, porosity, sat=1.0, volume=1.0, temp=25.0):
+    n = st.ncell
+    st["pri_molal"][:, :n] = m0
+    st["sec_molal"][:, :n] = sec0
+    st["pri_act_coef"][:, :n] = gp
+    st["sec_act_coef"][:, :n] = gc
+    st["total"][:, :n] = T_molal * den * 1.0e-3  # mol/kg -> mol/L
+    st["porosity"][0, :n] = porosity
+    st["saturation"][0, :n] = sat
+    st["den_kg"][0, :n] = den
+    st["volume"][0, :n] = volume
+    st["temperature"][0, :n] = temp
+
+
+# ---------------------------------------------------------------------------
+# calcite system on a 10x10x10 grid
+# ---------------------------------------------------------------------------
+def calcite_network(seed: int = 20261203) -> ReactionNetwork:
+    rng = _rng(seed)
+    # H+, HCO3-, Ca++  (calcite.dat:4-6: a0, Z)
+    Zp = [1.0, -1.0, 2.0]
+    a0p = [9.0, 4.0, 6.0]
+    names = ["OH-", "CO3--", "CO2(aq)", "CaHCO3+", "CaCO3(aq)"]
+    species = [[0], [0, 1], [0, 1], [2, 1], [0, 2, 1]]
+    stoich = [[-1.0], [-1.0, 1.0], [1.0, 1.0], [1.0, 1.0], [-1.0, 1.0, 1.0]]
+    h2o = [1.0, 0.0, -1.0, 0.0, 0.0]
+    Zc = [-1.0, -2.0, 0.0, 1.0, 0.0]
+    a0c = [3.5, 4.5, 3.0, 4.0, 3.0]
+    # reference composition: pH 7, HCO3- = Ca++ = 1e-3 molal, unit activity
+    xref = np.array([-7.0, -3.0, -3.0])
+    logK = np.zeros(5)
+    for k in range(5):
+        nu = np.zeros(3)
+        for j, v in zip(species[k], stoich[k]):
+            nu[j] += v
+        while True:
+            lk = rng.uniform(-14.0, 10.0)
+            lm = -lk + nu @ xref
+            if -14.0 <= lm <= -2.0:
+                logK[k] = lk
+                break
+    nu_cal = np.array([-1.0, 1.0, 1.0])
+    while True:
+        lk_cal = rng.uniform(-14.0, 10.0)
+        if -2.0 <= -lk_cal + nu_cal @ xref <= 2.0:
+            break
+    net = ReactionNetwork(
+        primary_species_names=["H+", "HCO3-", "Ca++"],
+        primary_spec_Z=Zp, primary_spec_a0=a0p,
+        eqcplx_names=names, eqcplx_species=species, eqcplx_stoich=stoich,
+        eqcplxh2ostoich=h2o, eqcplx_logK=logK, eqcplx_Z=Zc, eqcplx_a0=a0c,
+        kinmnrl_names=["Calcite"], kinmnrl_species=[[0, 2, 1]],
+        kinmnrl_stoich=[[-1.0, 1.0, 1.0]], kinmnrl_logK=[lk_cal],
+        kinmnrl_rate_constant=[1.0e-9], kinmnrl_molar_vol=[36.934e-6],
+    )
+    return net
+
+
+def calcite_workload(ncell: int = 1000, seed: int = 20261203, dt: float = 3600.0,
+                     wmax: float = 0.5, ld: Optional[int] = None) -> Workload:
+    net = calcite_network(seed)
+    rng = _rng(seed + 1)
+
+    def draw(k):
+        pH = rng.uniform(6.0, 8.0, k)
+        lhco3 = rng.uniform(np.log10(1e-6), np.log10(1e-2), k)
+        lca = rng.uniform(np.log10(1e-6), np.log10(1e-2), k)
+        return np.vstack([10.0 ** (-pH), 10.0 ** lhco3, 10.0 ** lca])
+
+    # no positivity filter here: at pH 6-8 with the drawn log K the H+ total
+    # is negative in almost every cell, and RStep then solves with
+    # total H+ = 1e-40 (reaction.F90:3617-3631), as the reference does
+    m0 = draw(ncell)
+    sec0, T0, gp, gc = _equilibrate_gamma(net, m0)
+    # neighbour along x of the 10x10x10 grid (periodic), mixing weight w
+    nb = (np.arange(ncell) + 1) % ncell
+    w = rng.uniform(0.0, wmax, ncell)
+    T = (1.0 - w) * T0 + w * T0[:, nb]
+    st = CellState(net, ncell, ld)
+    _fill_common(st, net, m0, sec0, gp, gc, T, WATER_DENSITY_25C, 0.3)
+    st["mnrl_volfrac"][0, :ncell] = 0.1
+    st["mnrl_area"][0, :ncell] = 1.0
+    notes = ("calcite system, log K drawn U[-14,10] under a sanity filter; "
+             "calcite log K drawn so log QK at pH 7 / 1e-3 molal is in [-2,2]; "
+             "initial calcite volume fraction 0.1")
+    return Workload("calcite_1000", net, bench_options(), st, dt, seed, notes)
+
+
+# ---------------------------------------------------------------------------
+# random 15-primary / 60-complex / 5-mineral network
+# ---------------------------------------------------------------------------
+def random_network(naq: int = 15, ncplx: int = 60, nmnrl: int = 5, seed: int = 20261204,
+                   logk_range=(-20.0, 15.0), logrk_range=(-12.0, -6.0),
+                   nbackground: int = 2) -> ReactionNetwork:
+    """Primaries 0..nbackground-1 are a background electrolyte (Z = +1, -1)
+    that sets the ionic strength and takes part in no reaction."""
+    rng = _rng(seed)
+    Z = rng.integers(-2, 3, naq).astype(float)
+    if nbackground >= 2:
+        Z[0], Z[1] = 1.0, -1.0
+    a0 = np.full(naq, 4.0)
+    active = np.arange(nbackground, naq)
+    xref = rng.uniform(-6.0, -2.0, naq)
+    species, stoich, logK, Zc, a0c = [], [], [], [], []
+    while len(species) < ncplx:
+        nsp = int(rng.integers(2, 5))
+        sp = rng.choice(active, size=nsp, replace=False)
+        nu = rng.integers(1, 4, nsp) * rng.choice([-1, 1], nsp)
+        if np.all(nu < 0):
+            continue
+        if abs(float(nu @ Z[sp])) > 3.0:
+            continue  # keep complex charges physical (|Z| <= 3)
+        lk = rng.uniform(*logk_range)
+        lm = -lk + float(nu @ xref[sp])
+        if not (-16.0 <= lm <= -3.0):
+            continue
+        # a negative coefficient may remove at most 2% of that primary at the
+        # reference composition, so totals stay positive (see
+        # _draw_positive_totals)
+        neg = nu < 0
+        if np.any(np.log10(-nu[neg]) + lm > xref[sp][neg] + np.log10(0.02)):
+            continue
+        species.append([int(s) for s in sp])
+        stoich.append([float(v) for v in nu])
+        logK.append(lk)
+        zk = float(nu @ Z[sp])
+        Zc.append(zk)
+        a0c.append(4.0 if zk != 0 else 3.0)
+    msp, mst, mlogK = [], [], []
+    while len(msp) < nmnrl:
+        nsp = int(rng.integers(2, 5))
+        sp = rng.choice(active, size=nsp, replace=False)
+        nu = rng.integers(1, 4, nsp) * rng.choice([-1, 1], nsp, p=[0.25, 0.75])
+        if np.all(nu < 0):
+            continue
+        lq = float(nu @ xref[sp])
+        msp.append([int(s) for s in sp])
+        mst.append([float(v) for v in nu])
+        mlogK.append(lq - rng.uniform(-1.0, 1.0))  # log QK at reference in [-1, 1]
+    rate = 10.0 ** rng.uniform(*logrk_range, nmnrl)
+    net = ReactionNetwork(
+        primary_species_names=["P%d" % i for i in range(naq)],
+        primary_spec_Z=Z, primary_spec_a0=a0,
+        eqcplx_names=["C%d" % k for k in range(ncplx)], eqcplx_species=species,
+        eqcplx_stoich=stoich, eqcplx_logK=logK, eqcplx_Z=Zc, eqcplx_a0=a0c,
+        kinmnrl_names=["M%d" % m for m in range(nmnrl)], kinmnrl_species=msp,
+        kinmnrl_stoich=mst, kinmnrl_logK=mlogK, kinmnrl_rate_constant=rate,
+        kinmnrl_molar_vol=np.full(nmnrl, 3.0e-5),
+    )
+    net._xref = xref
+    net._nbackground = nbackground
+    return net
+
+
+def random_network_workload(ncell: int = 1 << 20, naq: int = 15, ncplx: int = 60,
+                            nmnrl: int = 5, seed: int = 20261204, dt: float = 3600.0,
+                            wmax: float = 0.3, ld: Optional[int] = None,
+                            spread: float = 0.25) -> Workload:
+    net = random_network(naq, ncplx, nmnrl, seed)
+    rng = _rng(seed + 1)
+    nb_ = net._nbackground
+    m0 = _draw_positive_totals(
+        net, lambda k: 10.0 ** (net._xref[:, None] + rng.uniform(-spread, spread, (naq, k))),
+        ncell)
+    I_target = 10.0 ** rng.uniform(np.log10(1e-3), np.log10(0.5), ncell)
+    # background electrolyte sets the ionic strength
+    m0[:nb_] = 1e-6
+    for _ in range(2):
+        sec, _, gp, gc = _equilibrate_gamma(net, m0, passes=2)
+        Zp = net.primary_spec_Z[:, None]
+        Zc = net.eqcplx_Z[:, None]
+        I_other = 0.5 * ((m0[nb_:] * Zp[nb_:] ** 2).sum(0) + (sec * Zc ** 2).sum(0))
+        mb = np.maximum(I_target - I_other, 1e-3 * I_target)
+        m0[0] = mb
+        m0[1] = mb
+    sec0, T0, gp, gc = _equilibrate_gamma(net, m0)
+    nb = (np.arange(ncell) + 1) % ncell
+    w = rng.uniform(0.0, wmax, ncell)
+    T = (1.0 - w) * T0 + w * T0[:, nb]
+    st = CellState(net, ncell, ld)
+    _fill_common(st, net, m0, sec0, gp, gc, T, WATER_DENSITY_25C, 0.3)
+    # active-mineral subset per cell: count ~ U{0..nmnrl}
+    cnt = rng.integers(0, nmnrl + 1, ncell)
+    keys = rng.random((nmnrl, ncell))
+    rank = np.argsort(np.argsort(keys, axis=0), axis=0)
+    act = rank < cnt[None, :]
+    st["mnrl_area"][:, :ncell] = np.where(act, 1.0, 0.0)
+    st["mnrl_volfrac"][:, :ncell] = np.where(act, rng.uniform(0.0, 0.05, (nmnrl, ncell)), 0.0)
+    notes = ("random network: Z~U{-2..2} (P0/P1 background electrolyte Z=+1/-1 setting "
+             "I~logU[1e-3,0.5]), a0=4; complexes on 2-4 of P2..P14 with nu in "
+             "+-{1,2,3} and |Z|<=3, log K~U[-20,15] kept if log m_k at the reference "
+             "composition is in [-16,-3]; minerals log k~U[-12,-6], log QK at "
+             "reference in [-1,1], area 1 m^2/m^3 when active, active count "
+             "U{0..5} per cell")
+    return Workload("random_n%d_c%d_m%d" % (naq, ncplx, nmnrl), net, bench_options(), st,
+                    dt, seed, notes)
