@@ -17,6 +17,9 @@
  *  - SomDec stoichiometry that the reference stores in the shared sandbox
  *    object per cell visit (reaction_sandbox_somdec.F90:1817-1858) is kept
  *    per cell, starting from the Setup values.
+ *  - RStep gives up after ORC_MAX_RREACT_ATTEMPTS RReact calls; the
+ *    reference's sub-step loop has no bound (its cut counter can go
+ *    negative after runs of good sub-steps, reaction.F90:3721-3725).
  *  - Mineral rate prefactors (reaction_mineral.F90:806-870) and sorption,
  *    gas, surface complexation, general/microbial/radioactive-decay
  *    reactions are not part of this path.
@@ -731,10 +734,11 @@ int orc_rstep(const pfchem_network* net, const pfchem_options* opt,
     for (int i = 0; i < naq; ++i) guess[i] = c->total[i];
 
   double target_time = dt, cumulative_time = 0.0, tran_dt = dt;
-  int num_const = 0, num_cuts = 0, last_reason = 0;
+  int num_const = 0, num_cuts = 0, last_reason = 0, attempts = 0;
   orc_substep_record tmp;
   for (;;) {
     if (cumulative_time >= target_time) break;
+    if (++attempts > ORC_MAX_RREACT_ATTEMPTS) return last_reason ? last_reason : 1;
     int inner = 0, reason = 0;
     int ierr = orc_rreact(net, opt, c, guess, tran_dt, &inner, rec ? &tmp : NULL,
                           &reason);

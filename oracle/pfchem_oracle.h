@@ -24,6 +24,7 @@ extern "C" {
 #define ORC_MAXCPLX 512
 #define ORC_MAXMNRL 32
 #define ORC_MAXRXN 32    /* SomDec reactions                             */
+#define ORC_MAX_RREACT_ATTEMPTS 10000 /* RReact calls per RStep (safety bound) */
 
 /* one cell's auxvars: rt_auxvar + global_auxvar + material_auxvar subset
    (reactive_transport_aux.F90:21-74) */

@@ -1,0 +1,479 @@
+// pfchem_api.hip -- the C ABI (include/pfchem.h): context, network upload,
+// argument checks and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pfchem.h"
+#include "pfchem_device.hpp"
+#include "pfchem_kernels.hpp"
+
+using namespace pfchem;
+
+struct pfchem_ctx {
+  int device = 0;
+  int num_cus = 256;
+  DevNet dnet{};
+  void* dbuf = nullptr;  // device copy of the network arrays
+  unsigned long long* counter = nullptr;  // work cursor of the persistent kernels
+  DevNet* dnet_dev = nullptr;             // device copy of dnet
+  DevState* dstate_ring = nullptr;        // device slots for per-launch state descriptors
+  DevState* hstate_ring = nullptr;        // pinned host mirror the slots are copied from
+  int ring_next = 0;
+  LaunchCfg launch[3] = {{1, 128, 0}, {16, 256, 4}, {16, 256, 4}};
+  pfchem_network net{};   // sizes only (host pointers are not kept)
+  pfchem_options opt{};
+  bool has_somdec = false;
+  // host staging (pfchem_step_host)
+  void* stage = nullptr;
+  size_t stage_bytes = 0;
+  std::string err;
+};
+
+namespace {
+
+int fail(pfchem_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int hip_fail(pfchem_ctx* ctx, hipError_t e, const char* what) {
+  return fail(ctx, PFCHEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// byte buffer with 256-B aligned sub-arrays; offsets resolved after upload
+struct Packer {
+  std::vector<char> bytes;
+  size_t add(const void* p, size_t nbytes) {
+    size_t off = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(off + (nbytes ? nbytes : 1));
+    if (p && nbytes) std::memcpy(bytes.data() + off, p, nbytes);
+    return off;
+  }
+};
+
+const size_t kNone = ~size_t(0);
+// launches take their state descriptor from a ring of device slots; a slot
+// is rewritten (stream-ordered) only after kStateRing further launches
+const int kStateRing = 256;
+
+int check_reactions(const int32_t* ids, int n, int maxsp, int naq, const char* what,
+                    std::string& msg) {
+  for (int k = 0; k < n; ++k) {
+    const int32_t* row = ids + (size_t)k * (maxsp + 1);
+    if (row[0] < 0 || row[0] > maxsp) {
+      msg = std::string(what) + ": bad species count in reaction " + std::to_string(k);
+      return 1;
+    }
+    for (int i = 0; i < row[0]; ++i)
+      if (row[1 + i] < 0 || row[1 + i] >= naq) {
+        msg = std::string(what) + ": species id out of range in reaction " + std::to_string(k);
+        return 1;
+      }
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pfchem_strerror(int code) {
+  switch (code) {
+    case PFCHEM_OK: return "ok";
+    case PFCHEM_ERR_INVALID: return "invalid argument";
+    case PFCHEM_ERR_HIP: return "HIP runtime error";
+    case PFCHEM_ERR_UNSUPPORTED: return "unsupported network, option or size";
+    case PFCHEM_ERR_NOMEM: return "device allocation failed";
+    case PFCHEM_ERR_CELLS: return "one or more cells failed (see per-cell status)";
+    default: return "unknown pfchem error";
+  }
+}
+
+const char* pfchem_last_error(const pfchem_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int device,
+                  pfchem_ctx** out) {
+  if (!out) return PFCHEM_ERR_INVALID;
+  *out = nullptr;
+  if (!net || !opt) return PFCHEM_ERR_INVALID;
+  pfchem_ctx* ctx = new pfchem_ctx();
+  std::string msg;
+  const int naq = net->naqcomp, nimm = net->nimmobile;
+  if (naq < 0 || nimm < 0 || naq + nimm < 1) {
+    delete ctx;
+    return PFCHEM_ERR_INVALID;
+  }
+  if (naq + nimm > 64 || net->nkinmnrl > 16 || net->neqcplx > 4096) {
+    delete ctx;
+    return PFCHEM_ERR_UNSUPPORTED;
+  }
+  if ((naq && (!net->primary_spec_Z || !net->primary_spec_a0)) ||
+      (net->neqcplx > 0 && (!net->eqcplxspecid || !net->eqcplxstoich || !net->eqcplx_logK ||
+                            !net->eqcplx_Z || !net->eqcplx_a0)) ||
+      (net->nkinmnrl > 0 && (!net->kinmnrlspecid || !net->kinmnrlstoich || !net->kinmnrl_logK ||
+                             !net->kinmnrl_rate_constant || !net->kinmnrl_molar_vol)) ||
+      (net->ndecay_rxn > 0 && (!net->decayspecid || !net->decay_rate_constant))) {
+    delete ctx;
+    return PFCHEM_ERR_INVALID;
+  }
+  if (check_reactions(net->eqcplxspecid, net->neqcplx, net->eqcplx_maxsp, naq, "complex", msg) ||
+      check_reactions(net->kinmnrlspecid, net->nkinmnrl, net->kinmnrl_maxsp, naq, "mineral", msg)) {
+    delete ctx;
+    return PFCHEM_ERR_INVALID;
+  }
+  for (int r = 0; r < net->ndecay_rxn; ++r)
+    if (net->decayspecid[r] < 0 || net->decayspecid[r] >= nimm) {
+      delete ctx;
+      return PFCHEM_ERR_INVALID;
+    }
+  if (opt->act_coef_update_algorithm == PFCHEM_ACT_COEF_ALGORITHM_NEWTON ||
+      net->somdec != nullptr) {
+    delete ctx;
+    return PFCHEM_ERR_UNSUPPORTED;
+  }
+  if (opt->max_dlnC_rreact <= 0.0 || opt->maximum_reaction_iterations < 1 ||
+      opt->maximum_reaction_cuts < 0) {
+    delete ctx;
+    return PFCHEM_ERR_INVALID;
+  }
+
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return PFCHEM_ERR_HIP;
+  }
+  ctx->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    ctx->num_cus = cus;
+
+  const int nc = net->neqcplx, nm = net->nkinmnrl;
+  Packer P;
+  auto addd = [&](const double* p, size_t n) { return p ? P.add(p, n * sizeof(double)) : kNone; };
+  auto addi = [&](const int32_t* p, size_t n) { return p ? P.add(p, n * sizeof(int32_t)) : kNone; };
+  size_t oZ = addd(net->primary_spec_Z, naq), oa0 = addd(net->primary_spec_a0, naq);
+  size_t oci = addi(net->eqcplxspecid, (size_t)nc * (net->eqcplx_maxsp + 1));
+  size_t ocs = addd(net->eqcplxstoich, (size_t)nc * net->eqcplx_maxsp);
+  size_t och = addd(net->eqcplxh2ostoich, nc), ock = addd(net->eqcplx_logK, nc);
+  size_t ockc = addd(net->eqcplx_logKcoef, (size_t)nc * 5);
+  size_t ocZ = addd(net->eqcplx_Z, nc), oca = addd(net->eqcplx_a0, nc);
+  size_t omi = addi(net->kinmnrlspecid, (size_t)nm * (net->kinmnrl_maxsp + 1));
+  size_t oms = addd(net->kinmnrlstoich, (size_t)nm * net->kinmnrl_maxsp);
+  size_t omh = addd(net->kinmnrlh2ostoich, nm), omk = addd(net->kinmnrl_logK, nm);
+  size_t omkc = addd(net->kinmnrl_logKcoef, (size_t)nm * 5);
+  size_t omr = addd(net->kinmnrl_rate_constant, nm), omE = addd(net->kinmnrl_activation_energy, nm);
+  size_t omV = addd(net->kinmnrl_molar_vol, nm), omT = addd(net->kinmnrl_Temkin_const, nm);
+  size_t omS = addd(net->kinmnrl_min_scale_factor, nm), omP = addd(net->kinmnrl_affinity_power, nm);
+  size_t omt = addd(net->kinmnrl_affinity_threshold, nm), oml = addd(net->kinmnrl_rate_limiter, nm);
+  size_t omI = addi(net->kinmnrl_irreversible, nm);
+  size_t odi = addi(net->decayspecid, net->ndecay_rxn), odk = addd(net->decay_rate_constant, net->ndecay_rxn);
+
+  e = hipMalloc(&ctx->dbuf, P.bytes.size());
+  if (e != hipSuccess) {
+    delete ctx;
+    return PFCHEM_ERR_NOMEM;
+  }
+  e = hipMemcpy(ctx->dbuf, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&ctx->counter, 256);
+  if (e != hipSuccess) {
+    (void)hipFree(ctx->dbuf);
+    if (ctx->counter) (void)hipFree(ctx->counter);
+    delete ctx;
+    return PFCHEM_ERR_HIP;
+  }
+  char* base = (char*)ctx->dbuf;
+  auto pd = [&](size_t o) { return o == kNone ? (const double*)nullptr : (const double*)(base + o); };
+  auto pi = [&](size_t o) { return o == kNone ? (const int*)nullptr : (const int*)(base + o); };
+  DevNet& d = ctx->dnet;
+  d.naq = naq;
+  d.nimm = nimm;
+  d.ncomp = naq + nimm;
+  d.nc = nc;
+  d.nm = nm;
+  d.ndecay = net->ndecay_rxn;
+  d.cplx_ms = net->eqcplx_maxsp;
+  d.mnrl_ms = net->kinmnrl_maxsp;
+  d.h2o_aq_id = net->h2o_aq_id;
+  d.log_form = opt->use_log_formulation;
+  d.act_freq = opt->act_coef_update_frequency;
+  d.act_alg = opt->act_coef_update_algorithm;
+  d.act_h2o = opt->use_activity_h2o;
+  d.total_as_guess = opt->use_total_as_guess;
+  d.isothermal = opt->isothermal;
+  d.max_it = opt->maximum_reaction_iterations;
+  d.max_cuts = opt->maximum_reaction_cuts;
+  d.full_geochem = opt->use_full_geochemistry;
+  d.max_dlnC = opt->max_dlnC_rreact;
+  d.tol_abs = opt->max_residual_tolerance;
+  d.tol_rel = opt->max_rel_residual_tolerance;
+  d.tol_change = opt->max_relative_change_tolerance;
+  d.A = net->debyeA;
+  d.B = net->debyeB;
+  d.Bdot = net->debyeBdot;
+  d.has_cplx_h2o = och != kNone;
+  d.has_cplx_logKcoef = ockc != kNone;
+  d.has_mnrl_h2o = omh != kNone;
+  d.has_mnrl_logKcoef = omkc != kNone;
+  d.has_temkin = omT != kNone;
+  d.has_minscale = omS != kNone;
+  d.has_affpower = omP != kNone;
+  d.has_threshold = omt != kNone;
+  d.has_limiter = oml != kNone;
+  d.has_irrev = omI != kNone;
+  d.has_Ea = omE != kNone;
+  d.Zp = pd(oZ);
+  d.a0p = pd(oa0);
+  d.cplx_ids = pi(oci);
+  d.cplx_st = pd(ocs);
+  d.cplx_h2o = pd(och);
+  d.cplx_logK = pd(ock);
+  d.cplx_logKcoef = pd(ockc);
+  d.cplx_Z = pd(ocZ);
+  d.cplx_a0 = pd(oca);
+  d.mnrl_ids = pi(omi);
+  d.mnrl_st = pd(oms);
+  d.mnrl_h2o = pd(omh);
+  d.mnrl_logK = pd(omk);
+  d.mnrl_logKcoef = pd(omkc);
+  d.mnrl_k = pd(omr);
+  d.mnrl_Ea = pd(omE);
+  d.mnrl_Vm = pd(omV);
+  d.mnrl_temkin = pd(omT);
+  d.mnrl_minscale = pd(omS);
+  d.mnrl_affpower = pd(omP);
+  d.mnrl_threshold = pd(omt);
+  d.mnrl_limiter = pd(oml);
+  d.mnrl_irrev = pi(omI);
+  d.decay_id = pi(odi);
+  d.decay_k = pd(odk);
+  ctx->net = *net;
+  ctx->opt = *opt;
+  ctx->has_somdec = net->somdec != nullptr;
+  // default team width: next power of two >= ncomp (>= 4)
+  int L = 4;
+  while (L < d.ncomp) L *= 2;
+  for (int v = PFCHEM_KERNEL_BATCHED_FP64; v <= PFCHEM_KERNEL_BATCHED_MIXED; ++v) {
+    LaunchCfg c{L, 256, 4};
+    while (c.block_size > 64 && !team_supports(d, c, v == PFCHEM_KERNEL_BATCHED_MIXED))
+      c.block_size /= 2;
+    ctx->launch[v] = c;
+  }
+  if (hipMalloc(&ctx->dnet_dev, sizeof(DevNet)) != hipSuccess ||
+      hipMalloc(&ctx->dstate_ring, sizeof(DevState) * kStateRing) != hipSuccess ||
+      hipHostMalloc(&ctx->hstate_ring, sizeof(DevState) * kStateRing) != hipSuccess ||
+      hipMemcpy(ctx->dnet_dev, &ctx->dnet, sizeof(DevNet), hipMemcpyHostToDevice) != hipSuccess) {
+    pfchem_destroy(ctx);
+    return PFCHEM_ERR_NOMEM;
+  }
+  *out = ctx;
+  return PFCHEM_OK;
+}
+
+int pfchem_destroy(pfchem_ctx* ctx) {
+  if (!ctx) return PFCHEM_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->dbuf) (void)hipFree(ctx->dbuf);
+  if (ctx->counter) (void)hipFree(ctx->counter);
+  if (ctx->stage) (void)hipFree(ctx->stage);
+  if (ctx->dnet_dev) (void)hipFree(ctx->dnet_dev);
+  if (ctx->dstate_ring) (void)hipFree(ctx->dstate_ring);
+  if (ctx->hstate_ring) (void)hipHostFree(ctx->hstate_ring);
+  delete ctx;
+  return PFCHEM_OK;
+}
+
+static int to_dev_state(pfchem_ctx* ctx, const pfchem_state* st, int64_t ncell, DevState* d) {
+  if (!st) return fail(ctx, PFCHEM_ERR_INVALID, "null state");
+  if (st->ld < ncell) return fail(ctx, PFCHEM_ERR_INVALID, "state ld < ncell");
+  const DevNet& n = ctx->dnet;
+  if (!st->pri_molal || !st->total || !st->pri_act_coef || !st->porosity || !st->saturation ||
+      !st->den_kg || !st->volume)
+    return fail(ctx, PFCHEM_ERR_INVALID, "missing required state array");
+  if (n.nimm > 0 && !st->immobile) return fail(ctx, PFCHEM_ERR_INVALID, "missing immobile");
+  if (n.nc > 0 && (!st->sec_molal || !st->sec_act_coef))
+    return fail(ctx, PFCHEM_ERR_INVALID, "missing sec_molal / sec_act_coef");
+  if (n.nm > 0 && (!st->mnrl_volfrac || !st->mnrl_area))
+    return fail(ctx, PFCHEM_ERR_INVALID, "missing mineral arrays");
+  d->ld = st->ld;
+  d->pri_molal = st->pri_molal;
+  d->total = st->total;
+  d->immobile = st->immobile;
+  d->sec_molal = st->sec_molal;
+  d->pri_act_coef = st->pri_act_coef;
+  d->sec_act_coef = st->sec_act_coef;
+  d->ln_act_h2o = st->ln_act_h2o;
+  d->mnrl_volfrac = st->mnrl_volfrac;
+  d->mnrl_area = st->mnrl_area;
+  d->mnrl_rate = st->mnrl_rate;
+  d->porosity = st->porosity;
+  d->saturation = st->saturation;
+  d->den_kg = st->den_kg;
+  d->volume = st->volume;
+  d->temperature = st->temperature;
+  d->somdec_f_t = st->somdec_f_t;
+  d->somdec_f_w = st->somdec_f_w;
+  d->num_iterations = st->num_iterations;
+  d->num_sub_steps = st->num_sub_steps;
+  d->status = st->status;
+  return PFCHEM_OK;
+}
+
+int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
+                int variant, void* stream) {
+  if (!ctx) return PFCHEM_ERR_INVALID;
+  if (ncell < 0) return fail(ctx, PFCHEM_ERR_INVALID, "ncell < 0");
+  if (!(dt > 0.0)) return fail(ctx, PFCHEM_ERR_INVALID, "dt must be > 0");
+  if (variant < PFCHEM_KERNEL_PLAIN_FP64 || variant > PFCHEM_KERNEL_BATCHED_MIXED)
+    return fail(ctx, PFCHEM_ERR_INVALID, "unknown kernel variant");
+  DevState d;
+  int rc = to_dev_state(ctx, state, ncell, &d);
+  if (rc) return rc;
+  if (ncell == 0) return PFCHEM_OK;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  hipStream_t s = (hipStream_t)stream;
+  if (variant == PFCHEM_KERNEL_PLAIN_FP64) {
+    e = launch_plain_step(ctx->dnet, d, ncell, dt, s);
+  } else {
+    const bool mixed = variant == PFCHEM_KERNEL_BATCHED_MIXED;
+    const LaunchCfg& cfg = ctx->launch[variant];
+    if (!team_supports(ctx->dnet, cfg, mixed))
+      return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
+    DevState* slot = ctx->dstate_ring + ctx->ring_next;
+    DevState* hslot = ctx->hstate_ring + ctx->ring_next;
+    ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
+    *hslot = d;
+    e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+    e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cfg, ctx->counter,
+                         ctx->num_cus, s);
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+  return PFCHEM_OK;
+}
+
+int pfchem_update_activity(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                           void* stream) {
+  if (!ctx) return PFCHEM_ERR_INVALID;
+  DevState d;
+  int rc = to_dev_state(ctx, state, ncell, &d);
+  if (rc) return rc;
+  if (ncell == 0) return PFCHEM_OK;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = launch_activity_lag(ctx->dnet, d, ncell, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "activity launch");
+  return PFCHEM_OK;
+}
+
+int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                    const double* fixed_total, const double* fixed_immobile, double dt,
+                    double* res, void* stream) {
+  (void)ncell; (void)state; (void)fixed_total; (void)fixed_immobile; (void)dt; (void)res;
+  (void)stream;
+  return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "pfchem_residual not available yet");
+}
+
+int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg) {
+  if (!ctx || !cfg) return PFCHEM_ERR_INVALID;
+  if (variant < PFCHEM_KERNEL_PLAIN_FP64 || variant > PFCHEM_KERNEL_BATCHED_MIXED)
+    return fail(ctx, PFCHEM_ERR_INVALID, "unknown kernel variant");
+  if (cfg->block_size <= 0 || cfg->block_size % 64 || cfg->block_size > 1024 ||
+      cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64)
+    return fail(ctx, PFCHEM_ERR_INVALID, "bad launch shape");
+  LaunchCfg c{cfg->lanes_per_cell, cfg->block_size, cfg->blocks_per_cu};
+  if (variant != PFCHEM_KERNEL_PLAIN_FP64 &&
+      !team_supports(ctx->dnet, c, variant == PFCHEM_KERNEL_BATCHED_MIXED))
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
+  ctx->launch[variant] = c;
+  return PFCHEM_OK;
+}
+
+int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg) {
+  if (!ctx || !cfg || variant < 0 || variant > 2) return PFCHEM_ERR_INVALID;
+  cfg->lanes_per_cell = ctx->launch[variant].lanes_per_cell;
+  cfg->block_size = ctx->launch[variant].block_size;
+  cfg->blocks_per_cu = ctx->launch[variant].blocks_per_cu;
+  cfg->pad0 = 0;
+  return PFCHEM_OK;
+}
+
+int pfchem_step_host(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* h, double dt,
+                     int variant) {
+  if (!ctx || !h) return PFCHEM_ERR_INVALID;
+  if (ncell <= 0) return ncell == 0 ? PFCHEM_OK : PFCHEM_ERR_INVALID;
+  if (h->ld < ncell) return fail(ctx, PFCHEM_ERR_INVALID, "state ld < ncell");
+  const DevNet& n = ctx->dnet;
+  struct F { const void* src; int rows; int esz; bool out; size_t off; };
+  F f[] = {
+      {h->pri_molal, n.naq, 8, true, 0},       {h->total, n.naq, 8, true, 0},
+      {h->immobile, n.nimm, 8, true, 0},       {h->sec_molal, n.nc, 8, true, 0},
+      {h->pri_act_coef, n.naq, 8, true, 0},    {h->sec_act_coef, n.nc, 8, true, 0},
+      {h->ln_act_h2o, 1, 8, true, 0},          {h->mnrl_volfrac, n.nm, 8, true, 0},
+      {h->mnrl_area, n.nm, 8, false, 0},       {h->mnrl_rate, n.nm, 8, true, 0},
+      {h->porosity, 1, 8, false, 0},           {h->saturation, 1, 8, false, 0},
+      {h->den_kg, 1, 8, false, 0},             {h->volume, 1, 8, false, 0},
+      {h->temperature, 1, 8, false, 0},        {h->somdec_f_t, 1, 8, false, 0},
+      {h->somdec_f_w, 1, 8, false, 0},         {h->num_iterations, 1, 4, true, 0},
+      {h->num_sub_steps, 1, 4, true, 0},       {h->status, 1, 4, true, 0}};
+  const int nf = sizeof(f) / sizeof(f[0]);
+  size_t bytes = 0;
+  for (int i = 0; i < nf; ++i) {
+    f[i].off = bytes;
+    if (f[i].src && f[i].rows > 0)
+      bytes += (((size_t)f[i].rows * ncell * f[i].esz) + 255) & ~size_t(255);
+  }
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  if (bytes > ctx->stage_bytes) {
+    if (ctx->stage) (void)hipFree(ctx->stage);
+    ctx->stage = nullptr;
+    ctx->stage_bytes = 0;
+    if (hipMalloc(&ctx->stage, bytes) != hipSuccess) return fail(ctx, PFCHEM_ERR_NOMEM, "staging");
+    ctx->stage_bytes = bytes;
+  }
+  char* base = (char*)ctx->stage;
+  pfchem_state d{};
+  d.ld = ncell;
+  void** dp[] = {(void**)&d.pri_molal, (void**)&d.total, (void**)&d.immobile, (void**)&d.sec_molal,
+                 (void**)&d.pri_act_coef, (void**)&d.sec_act_coef, (void**)&d.ln_act_h2o,
+                 (void**)&d.mnrl_volfrac, (void**)&d.mnrl_area, (void**)&d.mnrl_rate,
+                 (void**)&d.porosity, (void**)&d.saturation, (void**)&d.den_kg,
+                 (void**)&d.volume, (void**)&d.temperature, (void**)&d.somdec_f_t,
+                 (void**)&d.somdec_f_w, (void**)&d.num_iterations, (void**)&d.num_sub_steps,
+                 (void**)&d.status};
+  for (int i = 0; i < nf; ++i) {
+    if (!f[i].src || f[i].rows <= 0) {
+      *dp[i] = nullptr;
+      continue;
+    }
+    *dp[i] = base + f[i].off;
+    // rows are ld-strided on the host, packed (ld = ncell) on the device
+    e = hipMemcpy2D(base + f[i].off, (size_t)ncell * f[i].esz, f[i].src, (size_t)h->ld * f[i].esz,
+                    (size_t)ncell * f[i].esz, f[i].rows, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(ctx, e, "stage in");
+  }
+  int rc = pfchem_step(ctx, ncell, &d, dt, variant, nullptr);
+  if (rc) return rc;
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(ctx, e, "step");
+  for (int i = 0; i < nf; ++i) {
+    if (!f[i].src || f[i].rows <= 0 || !f[i].out) continue;
+    e = hipMemcpy2D((void*)f[i].src, (size_t)h->ld * f[i].esz, base + f[i].off,
+                    (size_t)ncell * f[i].esz, (size_t)ncell * f[i].esz, f[i].rows,
+                    hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "stage out");
+  }
+  if (h->status)
+    for (int64_t c = 0; c < ncell; ++c)
+      if (h->status[c]) return fail(ctx, PFCHEM_ERR_CELLS, "cells failed");
+  return PFCHEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+// pfchem_device.hpp -- device-side network descriptor and common helpers for
+// the gfx950 chemistry-step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pfchem.h"
+
+namespace pfchem {
+
+// pflotran_constants.F90:84, 89; reactive_transport_aux.F90:19
+constexpr double kLogToLn = 2.30258509299;
+constexpr double kIdealGas = 8.31446;
+constexpr double kRtMinSaturation = 1.0e-40;
+
+// RReact calls allowed per RStep.  The reference's sub-step loop has no
+// bound (its cut counter can go negative after runs of good sub-steps,
+// reaction.F90:3721-3725); every kernel loop here must end.
+constexpr int kMaxRReactAttempts = 10000;
+
+enum FailReason : int { kOk = 0, kMaxIter = 1, kSingular = 2, kActivity = 3 };
+
+// Network + options as kernels see them: pointers into one device buffer,
+// passed by value as a kernel argument.  Field meanings follow
+// pfchem_network / pfchem_options (include/pfchem.h).
+struct DevNet {
+  int naq, nimm, ncomp, nc, nm, ndecay;
+  int cplx_ms, mnrl_ms;  // row lengths of the compressed reaction arrays
+  int h2o_aq_id;
+  // option flags
+  int log_form, act_freq, act_alg, act_h2o, total_as_guess, isothermal;
+  int max_it, max_cuts, full_geochem;
+  double max_dlnC, tol_abs, tol_rel, tol_change;
+  double A, B, Bdot;
+  // optional-array presence
+  int has_cplx_h2o, has_cplx_logKcoef, has_mnrl_h2o, has_mnrl_logKcoef;
+  int has_temkin, has_minscale, has_affpower, has_threshold, has_limiter;
+  int has_irrev, has_Ea;
+  // primaries
+  const double* Zp;
+  const double* a0p;
+  // complexes
+  const int* cplx_ids;  // [nc][cplx_ms+1], [k][0] = count
+  const double* cplx_st;
+  const double* cplx_h2o;
+  const double* cplx_logK;
+  const double* cplx_logKcoef;
+  const double* cplx_Z;
+  const double* cplx_a0;
+  // minerals
+  const int* mnrl_ids;
+  const double* mnrl_st;
+  const double* mnrl_h2o;
+  const double* mnrl_logK;
+  const double* mnrl_logKcoef;
+  const double* mnrl_k;
+  const double* mnrl_Ea;
+  const double* mnrl_Vm;
+  const double* mnrl_temkin;
+  const double* mnrl_minscale;
+  const double* mnrl_affpower;
+  const double* mnrl_threshold;
+  const double* mnrl_limiter;
+  const int* mnrl_irrev;
+  // immobile decay
+  const int* decay_id;
+  const double* decay_k;
+};
+
+// SoA per-cell state, device pointers (pfchem_state)
+struct DevState {
+  int64_t ld;
+  double* pri_molal;
+  double* total;
+  double* immobile;
+  double* sec_molal;
+  double* pri_act_coef;
+  double* sec_act_coef;
+  double* ln_act_h2o;
+  double* mnrl_volfrac;
+  const double* mnrl_area;
+  double* mnrl_rate;
+  const double* porosity;
+  const double* saturation;
+  const double* den_kg;
+  const double* volume;
+  const double* temperature;
+  const double* somdec_f_t;
+  const double* somdec_f_w;
+  int32_t* num_iterations;
+  int32_t* num_sub_steps;
+  int32_t* status;
+};
+
+__device__ __forceinline__ double sat_of(const DevState& s, int64_t c) {
+  return s.saturation[c];
+}
+
+// ReactionInterpolateLogK (reaction_aux.F90:1285-1312) for one entry
+__device__ __forceinline__ double interp_logk(const double* b, double temp) {
+  double tk = temp + 273.15;
+  return b[0] * log(tk) + b[1] + b[2] * tk + b[3] / tk + b[4] / (tk * tk);
+}
+
+// Debye-Hueckel ln(gamma) * 1/ln10 -> gamma (reaction.F90:4586-4592)
+__device__ __forceinline__ double dh_gamma(double z, double a0, double sqrtI,
+                                           double I, const DevNet& n) {
+  return exp((-z * z * sqrtI * n.A / (1.0 + a0 * n.B * sqrtI) + n.Bdot * I) *
+             kLogToLn);
+}
+
+}  // namespace pfchem

@@ -1,0 +1,33 @@
+// pfchem_kernels.hpp -- host-side launchers of the chemistry-step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pfchem_device.hpp"
+
+namespace pfchem {
+
+// persistent-kernel launch shape (see pfchem_launch in include/pfchem.h)
+struct LaunchCfg {
+  int lanes_per_cell;
+  int block_size;
+  int blocks_per_cu;
+};
+
+int plain_max_ncomp();
+hipError_t launch_plain_step(const DevNet& n, const DevState& s, int64_t ncell,
+                             double dt, hipStream_t stream);
+hipError_t launch_activity_lag(const DevNet& n, const DevState& s, int64_t ncell,
+                               hipStream_t stream);
+
+// batched (team) kernel: `counter` is a device int64 work cursor that the
+// launcher zeroes with a fill kernel (no memset node) before the launch.
+int team_max_ncomp();
+int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
+// dn / ds: device copies of the network / state descriptors
+hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,
+                            int64_t ncell, double dt, bool mixed, const LaunchCfg& cfg,
+                            unsigned long long* counter, int num_cus,
+                            hipStream_t stream);
+
+}  // namespace pfchem

@@ -1,0 +1,518 @@
+// pfchem_plain.hip -- plain FP64 reaction step: one cell per thread.
+//
+// The straightforward GPU form of RStep (reaction.F90:3564-3738): every
+// thread runs the whole sub-step / Newton loop of one cell with runtime-sized
+// loops over the network and per-thread arrays (compiler-placed in scratch).
+// No work stealing: a wavefront runs as long as its slowest cell.  This is
+// the FP64 reference point the batched kernels are measured against.
+#include "pfchem_device.hpp"
+#include "pfchem_kernels.hpp"
+
+namespace pfchem {
+namespace {
+
+constexpr int kMaxMnrl = 16;
+
+template <int MAXN>
+struct PlainCell {
+  double x[MAXN];        // pri_molal (aq) then immobile
+  double total[MAXN];    // rt_auxvar%total(:,1) (aq part used)
+  double imm[MAXN];      // rt_auxvar%immobile
+  double gp[MAXN];       // pri_act_coef
+  double J[MAXN * MAXN]; // row-major J(i,j); dtotal shares the aq block
+  double res[MAXN];
+  double vf[kMaxMnrl], rate[kMaxMnrl], logKm[kMaxMnrl];
+  double ln_act_h2o;
+  double por, sat, den, vol, temp;
+};
+
+// RTotalAqueous (reaction.F90:4665-4759): totals, secondary molalities
+// (written to the SoA state) and dtotal into the aq block of J (not yet
+// multiplied by the accumulation factor).
+template <int MAXN>
+__device__ void plain_total(const DevNet& n, const DevState& s, int64_t c,
+                            PlainCell<MAXN>& w, int want_dtotal) {
+  const int naq = n.naq, N = n.ncomp, ms = n.cplx_ms;
+  double lnc[MAXN], lna[MAXN];
+  for (int i = 0; i < naq; ++i) {
+    lnc[i] = log(w.x[i]);
+    lna[i] = lnc[i] + log(w.gp[i]);
+    w.total[i] = w.x[i];
+  }
+  if (want_dtotal) {
+    for (int i = 0; i < naq; ++i)
+      for (int j = 0; j < naq; ++j) w.J[i * N + j] = (i == j) ? 1.0 : 0.0;
+  }
+  for (int k = 0; k < n.nc; ++k) {
+    double logK = n.cplx_logK[k];
+    if (!n.isothermal && n.has_cplx_logKcoef)
+      logK = interp_logk(n.cplx_logKcoef + 5 * k, w.temp);
+    double lnQK = -logK * kLogToLn;
+    if (n.has_cplx_h2o && n.cplx_h2o[k] != 0.0) lnQK += n.cplx_h2o[k] * w.ln_act_h2o;
+    const int* ids = n.cplx_ids + k * (ms + 1);
+    const double* st = n.cplx_st + k * ms;
+    const int ncs = ids[0];
+    for (int i = 0; i < ncs; ++i) lnQK += st[i] * lna[ids[1 + i]];
+    const double gk = s.sec_act_coef[k * s.ld + c];
+    const double mk = exp(lnQK) / gk;
+    s.sec_molal[k * s.ld + c] = mk;
+    for (int i = 0; i < ncs; ++i) w.total[ids[1 + i]] += st[i] * mk;
+    if (want_dtotal) {
+      for (int j = 0; j < ncs; ++j) {
+        const int jc = ids[1 + j];
+        const double t = st[j] * exp(lnQK - lnc[jc]) / gk;
+        for (int i = 0; i < ncs; ++i) w.J[ids[1 + i] * N + jc] += st[i] * t;
+      }
+    }
+  }
+  const double dkl = w.den * 1.0e-3;
+  for (int i = 0; i < naq; ++i) w.total[i] *= dkl;
+  if (want_dtotal)
+    for (int i = 0; i < naq; ++i)
+      for (int j = 0; j < naq; ++j) w.J[i * N + j] *= dkl;
+}
+
+// RKineticMineral (reaction_mineral.F90:647-1078), default path
+template <int MAXN>
+__device__ void plain_minerals(const DevNet& n, const DevState& s, int64_t c,
+                               PlainCell<MAXN>& w, double* res, double* J,
+                               int deriv) {
+  const int naq = n.naq, N = n.ncomp, ms = n.mnrl_ms;
+  double lnc[MAXN], lna[MAXN];
+  for (int i = 0; i < naq; ++i) {
+    lnc[i] = log(w.x[i]);
+    lna[i] = lnc[i] + log(w.gp[i]);
+  }
+  for (int m = 0; m < n.nm; ++m) w.rate[m] = 0.0;
+  for (int m = 0; m < n.nm; ++m) {
+    const int* ids = n.mnrl_ids + m * (ms + 1);
+    const double* st = n.mnrl_st + m * ms;
+    const int ncs = ids[0];
+    double lnQK = -w.logKm[m] * kLogToLn;
+    if (n.has_mnrl_h2o && n.mnrl_h2o[m] != 0.0) lnQK += n.mnrl_h2o[m] * w.ln_act_h2o;
+    for (int i = 0; i < ncs; ++i) lnQK += st[i] * lna[ids[1 + i]];
+    const double QK = exp(lnQK);
+    double aff;
+    if (n.has_temkin) {
+      aff = n.has_minscale ? 1.0 - pow(QK, 1.0 / (n.mnrl_minscale[m] * n.mnrl_temkin[m]))
+                           : 1.0 - pow(QK, 1.0 / n.mnrl_temkin[m]);
+    } else if (n.has_minscale) {
+      aff = 1.0 - pow(QK, 1.0 / n.mnrl_minscale[m]);
+    } else {
+      aff = 1.0 - QK;
+    }
+    const double sgn = copysign(1.0, aff);
+    if (!(w.vf[m] > 0.0 || sgn < 0.0)) continue;
+    if (n.has_irrev && n.mnrl_irrev[m] == 1 && sgn < 0.0) continue;
+    if (n.has_threshold && n.mnrl_threshold[m] > 0.0 && sgn < 0.0 &&
+        QK < n.mnrl_threshold[m])
+      continue;
+    const double lim = n.has_limiter ? n.mnrl_limiter[m] : 0.0;
+    if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
+    double arr = 1.0;
+    if (n.has_Ea && n.mnrl_Ea[m] > 0.0)
+      arr = exp(n.mnrl_Ea[m] / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (w.temp + 273.15)));
+    const double kr = n.mnrl_k[m] * arr;
+    double Im_const = -s.mnrl_area[m * s.ld + c];
+    if (n.has_minscale) Im_const = Im_const / n.mnrl_minscale[m];
+    double Im = n.has_affpower
+                    ? Im_const * sgn * pow(fabs(aff), n.mnrl_affpower[m]) * kr
+                    : Im_const * sgn * fabs(aff) * kr;
+    w.rate[m] = Im;
+    Im_const *= w.vol;
+    Im *= w.vol;
+    for (int i = 0; i < ncs; ++i) res[ids[1 + i]] += st[i] * Im;
+    if (!deriv) continue;
+    double dIm_dQK = n.has_affpower ? -Im * n.mnrl_affpower[m] / fabs(aff) : -Im_const * kr;
+    if (n.has_temkin) {
+      const double e = n.has_minscale ? n.mnrl_minscale[m] * n.mnrl_temkin[m] : n.mnrl_temkin[m];
+      dIm_dQK = dIm_dQK * (1.0 / e) / QK * (1.0 - aff);
+    } else if (n.has_minscale) {
+      dIm_dQK = dIm_dQK * (1.0 / n.mnrl_minscale[m]) / QK * (1.0 - aff);
+    }
+    double fac = 1.0;
+    if (lim > 0.0) {
+      const double den = 1.0 + (1.0 - aff) / lim;
+      fac = (1.0 + QK / lim / den) / den;
+    }
+    for (int j = 0; j < ncs; ++j) {
+      const int jc = ids[1 + j];
+      const double dQK_dmj = st[j] * QK * exp(-lnc[jc]) * w.den * 1.0e-3;
+      for (int i = 0; i < ncs; ++i) {
+        if (lim > 0.0)
+          J[ids[1 + i] * N + jc] += st[i] * dIm_dQK * fac * dQK_dmj;
+        else
+          J[ids[1 + i] * N + jc] += st[i] * dIm_dQK * dQK_dmj;
+      }
+    }
+  }
+}
+
+// LUDecomposition1 (utility.F90:597-688) + LUBackSubstitution (:692-735)
+template <int MAXN>
+__device__ int plain_lu_solve(double* a, int N, double* b) {
+  double vv[MAXN];
+  int indx[MAXN];
+  int imax = 0;
+  for (int i = 0; i < N; ++i) {
+    double amax = 0.0;
+    for (int j = 0; j < N; ++j) amax = fmax(amax, fabs(a[i * N + j]));
+    if (amax <= 0.0) return 1;
+    vv[i] = 1.0 / amax;
+  }
+  for (int j = 0; j < N; ++j) {
+    for (int i = 0; i < j; ++i) {
+      double sum = a[i * N + j];
+      for (int k = 0; k < i; ++k) sum -= a[i * N + k] * a[k * N + j];
+      a[i * N + j] = sum;
+    }
+    double amax = 0.0;
+    for (int i = j; i < N; ++i) {
+      double sum = a[i * N + j];
+      for (int k = 0; k < j; ++k) sum -= a[i * N + k] * a[k * N + j];
+      a[i * N + j] = sum;
+      const double dum = vv[i] * fabs(sum);
+      if (dum >= amax) {
+        imax = i;
+        amax = dum;
+      }
+    }
+    if (j != imax) {
+      for (int k = 0; k < N; ++k) {
+        const double t = a[imax * N + k];
+        a[imax * N + k] = a[j * N + k];
+        a[j * N + k] = t;
+      }
+      vv[imax] = vv[j];
+    }
+    indx[j] = imax;
+    if (a[j * N + j] == 0.0) a[j * N + j] = 1.0e-20;
+    if (j != N - 1) {
+      const double dum = 1.0 / a[j * N + j];
+      for (int i = j + 1; i < N; ++i) a[i * N + j] *= dum;
+    }
+  }
+  int ii = -1;
+  for (int i = 0; i < N; ++i) {
+    const int ll = indx[i];
+    double sum = b[ll];
+    b[ll] = b[i];
+    if (ii >= 0) {
+      for (int j = ii; j < i; ++j) sum -= a[i * N + j] * b[j];
+    } else if (sum != 0.0) {
+      ii = i;
+    }
+    b[i] = sum;
+  }
+  for (int i = N - 1; i >= 0; --i) {
+    double sum = b[i];
+    for (int j = i + 1; j < N; ++j) sum -= a[i * N + j] * b[j];
+    b[i] = sum / a[i * N + i];
+  }
+  return 0;
+}
+
+// RActivityCoefficients LAG (reaction.F90:4555-4611) with the cell's
+// current pri/sec molalities; sec_act_coef written to the SoA state.
+template <int MAXN>
+__device__ void plain_activity_lag(const DevNet& n, const DevState& s, int64_t c,
+                                   PlainCell<MAXN>& w) {
+  double I = 0.0, sum_pri = 0.0, sum_sec = 0.0;
+  for (int i = 0; i < n.naq; ++i) {
+    I += w.x[i] * n.Zp[i] * n.Zp[i];
+    if (i != n.h2o_aq_id) sum_pri += w.x[i];
+  }
+  for (int k = 0; k < n.nc; ++k) {
+    const double m = s.sec_molal[k * s.ld + c];
+    I += m * n.cplx_Z[k] * n.cplx_Z[k];
+  }
+  I *= 0.5;
+  const double sI = sqrt(I);
+  for (int i = 0; i < n.naq; ++i)
+    w.gp[i] = fabs(n.Zp[i]) > 1.0e-10 ? dh_gamma(n.Zp[i], n.a0p[i], sI, I, n) : 1.0;
+  for (int k = 0; k < n.nc; ++k) {
+    s.sec_act_coef[k * s.ld + c] =
+        fabs(n.cplx_Z[k]) > 1.0e-10 ? dh_gamma(n.cplx_Z[k], n.cplx_a0[k], sI, I, n) : 1.0;
+    sum_sec += s.sec_molal[k * s.ld + c];
+  }
+  if (n.act_h2o) {
+    const double a = 1.0 - 0.017 * (sum_pri + sum_sec);
+    w.ln_act_h2o = a > 0.0 ? log(a) : 0.0;
+  }
+}
+
+// RReact (reaction.F90:3742-4055) for the plain kernel.  Returns the fail
+// reason (0 = converged); *nit receives the Newton iterations.
+template <int MAXN>
+__device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
+                            PlainCell<MAXN>& w, const double* guess, double dt,
+                            int* nit) {
+  const int naq = n.naq, nimm = n.nimm, N = n.ncomp;
+  double fixed[MAXN], initial[MAXN], upd[MAXN], prev[MAXN];
+  double r0 = 0.0;
+  *nit = 0;
+  if (!n.isothermal && n.has_mnrl_logKcoef)
+    for (int m = 0; m < n.nm; ++m) w.logKm[m] = interp_logk(n.mnrl_logKcoef + 5 * m, w.temp);
+  // RTAccumulation of total^k / immobile^k
+  const bool wet = !(w.sat < kRtMinSaturation);
+  const double psv = w.por * w.sat * 1000.0 * w.vol;
+  for (int i = 0; i < N; ++i) fixed[i] = 0.0;
+  if (wet) {
+    for (int i = 0; i < naq; ++i) fixed[i] = psv * w.total[i];
+    for (int i = 0; i < nimm; ++i) fixed[naq + i] += w.imm[i] * w.vol;
+  }
+  for (int i = 0; i < naq; ++i) initial[i] = w.total[i];
+  for (int i = 0; i < nimm; ++i) initial[naq + i] = w.imm[i];
+  for (int i = 0; i < naq; ++i) w.x[i] = guess[i];
+  for (int i = 0; i < nimm; ++i) {
+    w.imm[i] = guess[naq + i];
+    w.x[naq + i] = guess[naq + i];
+  }
+  for (;;) {
+    ++*nit;
+    if (n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER) plain_activity_lag(n, s, c, w);
+    plain_total(n, s, c, w, 1);
+    if (*nit > n.max_it) {
+      for (int i = 0; i < naq; ++i) w.total[i] = initial[i];
+      for (int i = 0; i < nimm; ++i) {
+        w.imm[i] = initial[naq + i];
+        w.x[naq + i] = initial[naq + i];
+      }
+      return kMaxIter;
+    }
+    // RTAccumulation / RTAccumulationDerivative
+    for (int i = 0; i < N; ++i) w.res[i] = 0.0;
+    if (wet) {
+      const double psvd = psv / dt;
+      for (int i = 0; i < naq; ++i) {
+        w.res[i] = psv * w.total[i];
+        for (int j = 0; j < naq; ++j) w.J[i * N + j] *= psvd;
+        for (int j = naq; j < N; ++j) w.J[i * N + j] = 0.0;
+      }
+      for (int i = 0; i < nimm; ++i) {
+        const int d = naq + i;
+        w.res[d] += w.imm[i] * w.vol;
+        for (int j = 0; j < N; ++j) w.J[d * N + j] = 0.0;
+        w.J[d * N + d] = w.vol / dt;
+      }
+    } else {
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) w.J[i * N + j] = (i == j) ? 1.0 : 0.0;
+    }
+    for (int i = 0; i < N; ++i) w.res[i] = (w.res[i] - fixed[i]) / dt;
+    // RReaction (reaction.F90:4059-4130)
+    if (wet) {
+      if (n.nm > 0) plain_minerals(n, s, c, w, w.res, w.J, 1);
+      for (int r = 0; r < n.ndecay; ++r) {
+        const int ic = n.decay_id[r];
+        const double kc = n.decay_k[r] * w.vol;
+        const int d = naq + ic;
+        w.res[d] += kc * w.imm[ic];
+        w.J[d * N + d] += kc;
+      }
+    }
+    double nr = 0.0, mx = 0.0;
+    for (int i = 0; i < N; ++i) {
+      nr += w.res[i] * w.res[i];
+      mx = fmax(mx, fabs(w.res[i]));
+    }
+    nr = sqrt(nr);
+    if (*nit == 1) r0 = nr;
+    if (mx < n.tol_abs) break;
+    if (nr / r0 < n.tol_rel) break;
+    // RSolve (reaction.F90:5457-5516), back-substituting form
+    for (int i = 0; i < N; ++i) prev[i] = w.x[i];
+    for (int i = 0; i < N; ++i) {
+      double nm = 0.0;
+      for (int j = 0; j < N; ++j) nm = fmax(nm, fabs(w.J[i * N + j]));
+      nm = 1.0 / fmax(nm, 1.0);
+      upd[i] = w.res[i] * nm;
+      for (int j = 0; j < N; ++j) w.J[i * N + j] *= nm;
+    }
+    if (n.log_form)
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) w.J[i * N + j] *= prev[j];
+    if (plain_lu_solve<MAXN>(w.J, N, upd)) return kSingular;
+    double mrc = -1.0;
+    if (n.log_form) {
+      for (int i = 0; i < N; ++i) {
+        const double u = copysign(1.0, upd[i]) * fmin(fabs(upd[i]), n.max_dlnC);
+        upd[i] = prev[i] * exp(-u);
+      }
+    } else {
+      double mr = 1.0e308;
+      for (int i = 0; i < N; ++i)
+        if (prev[i] <= upd[i]) mr = fmin(mr, fabs(prev[i] / upd[i]));
+      for (int i = 0; i < N; ++i)
+        upd[i] = prev[i] - (mr < 1.0 ? upd[i] * mr * 0.99 : upd[i]);
+    }
+    for (int i = 0; i < N; ++i) {
+      const double rc = fabs((upd[i] - prev[i]) / prev[i]);
+      if (rc == rc) mrc = fmax(mrc, rc);
+    }
+    if (mrc >= 0.0 && mrc < n.tol_change) break;
+    for (int i = 0; i < N; ++i) w.x[i] = upd[i];
+    for (int i = 0; i < nimm; ++i) w.imm[i] = upd[naq + i];
+  }
+  plain_total(n, s, c, w, 0);
+  return kOk;
+}
+
+template <int MAXN>
+__global__ void __launch_bounds__(128)
+plain_step_kernel(DevNet n, DevState s, int64_t ncell, double dt) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int naq = n.naq, nimm = n.nimm, N = n.ncomp;
+  PlainCell<MAXN> w;
+  for (int i = 0; i < naq; ++i) {
+    w.x[i] = s.pri_molal[i * s.ld + c];
+    w.total[i] = s.total[i * s.ld + c];
+    w.gp[i] = s.pri_act_coef[i * s.ld + c];
+  }
+  for (int i = 0; i < nimm; ++i) {
+    w.imm[i] = s.immobile[i * s.ld + c];
+    w.x[naq + i] = w.imm[i];
+  }
+  for (int m = 0; m < n.nm; ++m) {
+    w.vf[m] = s.mnrl_volfrac[m * s.ld + c];
+    w.logKm[m] = n.mnrl_logK[m];
+    w.rate[m] = 0.0;
+  }
+  w.ln_act_h2o = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
+  w.por = s.porosity[c];
+  w.sat = s.saturation[c];
+  w.den = s.den_kg[c];
+  w.vol = s.volume[c];
+  w.temp = s.temperature ? s.temperature[c] : 25.0;
+
+  int nsub = 0, nit_total = 0, status = 0;
+  if (!n.full_geochem) {
+    for (int i = 0; i < naq; ++i) w.x[i] = w.total[i] / w.den * 1.0e3;
+  } else {
+    // RStep (reaction.F90:3564-3738)
+    double guess[MAXN], small_val[MAXN];
+    uint64_t small = 0;
+    for (int i = 0; i < naq; ++i)
+      if (w.total[i] <= 1.0e-40) {
+        small |= 1ull << i;
+        small_val[i] = w.total[i];
+        w.total[i] = 1.0e-40;
+      }
+    for (int i = 0; i < nimm; ++i)
+      if (w.imm[i] <= 1.0e-40) {
+        small |= 1ull << (naq + i);
+        small_val[naq + i] = w.imm[i];
+        w.imm[i] = 1.0e-40;
+      }
+    for (int i = 0; i < N; ++i) guess[i] = w.x[i];
+    if (n.total_as_guess)
+      for (int i = 0; i < naq; ++i) guess[i] = w.total[i];
+    double target = dt, cum = 0.0, tdt = dt;
+    int nconst = 0, ncuts = 0, attempts = 0, last_why = 0;
+    for (;;) {
+      if (cum >= target) break;
+      if (++attempts > kMaxRReactAttempts) {
+        status = last_why ? last_why : kMaxIter;
+        break;
+      }
+      int inner = 0;
+      const int why = plain_rreact<MAXN>(n, s, c, w, guess, tdt, &inner);
+      nit_total += inner;
+      if (why) {
+        last_why = why;
+        ++ncuts;
+        if (ncuts > n.max_cuts) {
+          status = why;
+          break;
+        }
+        tdt *= 0.5;
+        nconst = 0;
+      } else {
+        // RUpdateKineticState -> MineralUpdateKineticState
+        if (n.nm > 0) {
+          double rdum[MAXN];
+          for (int i = 0; i < N; ++i) rdum[i] = 0.0;
+          plain_minerals(n, s, c, w, rdum, (double*)nullptr, 0);
+          for (int m = 0; m < n.nm; ++m) {
+            w.vf[m] += w.rate[m] * n.mnrl_Vm[m] * tdt;
+            if (w.vf[m] < 0.0) w.vf[m] = 0.0;
+          }
+        }
+        cum += tdt;
+        ++nsub;
+        ++nconst;
+        for (int i = 0; i < N; ++i) guess[i] = w.x[i];
+        if (nconst >= 4) {
+          --ncuts;
+          tdt = fmin(2.0 * tdt, target - cum);
+        }
+      }
+    }
+    if (!status) {
+      for (int i = 0; i < naq; ++i)
+        if (small & (1ull << i)) w.total[i] = small_val[i];
+      for (int i = 0; i < nimm; ++i)
+        if (small & (1ull << (naq + i))) w.imm[i] = small_val[naq + i];
+    }
+  }
+  for (int i = 0; i < naq; ++i) {
+    s.pri_molal[i * s.ld + c] = w.x[i];
+    s.total[i * s.ld + c] = w.total[i];
+  }
+  for (int i = 0; i < nimm; ++i) s.immobile[i * s.ld + c] = w.imm[i];
+  for (int m = 0; m < n.nm; ++m) {
+    s.mnrl_volfrac[m * s.ld + c] = w.vf[m];
+    if (s.mnrl_rate) s.mnrl_rate[m * s.ld + c] = w.rate[m];
+  }
+  if (s.num_iterations) s.num_iterations[c] = nit_total;
+  if (s.num_sub_steps) s.num_sub_steps[c] = nsub;
+  if (s.status) s.status[c] = status;
+}
+
+template <int MAXN>
+__global__ void __launch_bounds__(256)
+activity_kernel(DevNet n, DevState s, int64_t ncell) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  PlainCell<MAXN> w;
+  for (int i = 0; i < n.naq; ++i) w.x[i] = s.pri_molal[i * s.ld + c];
+  w.ln_act_h2o = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
+  plain_activity_lag(n, s, c, w);
+  for (int i = 0; i < n.naq; ++i) s.pri_act_coef[i * s.ld + c] = w.gp[i];
+  if (s.ln_act_h2o && n.act_h2o) s.ln_act_h2o[c] = w.ln_act_h2o;
+}
+
+}  // namespace
+
+int plain_max_ncomp() { return 64; }
+
+hipError_t launch_plain_step(const DevNet& n, const DevState& s, int64_t ncell,
+                             double dt, hipStream_t stream) {
+  const int block = 128;
+  const dim3 grid((unsigned)((ncell + block - 1) / block));
+  if (n.ncomp <= 4)
+    plain_step_kernel<4><<<grid, block, 0, stream>>>(n, s, ncell, dt);
+  else if (n.ncomp <= 8)
+    plain_step_kernel<8><<<grid, block, 0, stream>>>(n, s, ncell, dt);
+  else if (n.ncomp <= 16)
+    plain_step_kernel<16><<<grid, block, 0, stream>>>(n, s, ncell, dt);
+  else if (n.ncomp <= 32)
+    plain_step_kernel<32><<<grid, block, 0, stream>>>(n, s, ncell, dt);
+  else
+    plain_step_kernel<64><<<grid, block, 0, stream>>>(n, s, ncell, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_activity_lag(const DevNet& n, const DevState& s, int64_t ncell,
+                               hipStream_t stream) {
+  const int block = 256;
+  const dim3 grid((unsigned)((ncell + block - 1) / block));
+  if (n.naq <= 16)
+    activity_kernel<16><<<grid, block, 0, stream>>>(n, s, ncell);
+  else
+    activity_kernel<64><<<grid, block, 0, stream>>>(n, s, ncell);
+  return hipGetLastError();
+}
+
+}  // namespace pfchem

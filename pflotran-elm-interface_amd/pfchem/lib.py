@@ -1,0 +1,158 @@
+"""ctypes binding of libpfchem.so (the HIP chemistry step, include/pfchem.h).
+
+The library must be built (``__graft_entry__.build()`` or
+``make -C pflotran-elm-interface_amd/csrc``); loading fails loudly
+otherwise -- there is no CPU fallback.  torch is used only for device memory
+and streams: it must be imported before the library is loaded so both share
+one HIP runtime.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import abi
+from .network import CellState, ReactionNetwork, SolverOptions, state_fields
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libpfchem.so")
+
+_lib = None
+
+
+class PfchemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("pfchem error %d (%s): %s" % (code, abi_strerror(code), msg))
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libpfchem.so not built at %s (run __graft_entry__.build())" % path)
+    import torch  # noqa: F401  (shares torch's HIP runtime)
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.pfchem_create.argtypes = [vp, vp, C.c_int, C.POINTER(vp)]
+    L.pfchem_destroy.argtypes = [vp]
+    L.pfchem_strerror.restype = C.c_char_p
+    L.pfchem_strerror.argtypes = [C.c_int]
+    L.pfchem_last_error.restype = C.c_char_p
+    L.pfchem_last_error.argtypes = [vp]
+    L.pfchem_step.argtypes = [vp, C.c_int64, vp, C.c_double, C.c_int, vp]
+    L.pfchem_step_host.argtypes = [vp, C.c_int64, vp, C.c_double, C.c_int]
+    L.pfchem_update_activity.argtypes = [vp, C.c_int64, vp, vp]
+    L.pfchem_residual.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp]
+    L.pfchem_set_launch.argtypes = [vp, C.c_int, vp]
+    L.pfchem_get_launch.argtypes = [vp, C.c_int, vp]
+    _lib = L
+    return L
+
+
+def abi_strerror(code):
+    try:
+        return load().pfchem_strerror(code).decode()
+    except Exception:  # pragma: no cover
+        return "?"
+
+
+class DeviceState:
+    """SoA state in device memory (torch tensors shaped (rows, ld))."""
+
+    def __init__(self, host: CellState, device="cuda"):
+        import torch
+        self.net = host.net
+        self.ncell = host.ncell
+        self.ld = host.ld
+        self.fields = host.fields
+        self.t: Dict[str, "torch.Tensor"] = {}
+        for name, a in host.arrays.items():
+            self.t[name] = torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+    def to_host(self) -> CellState:
+        out = CellState.__new__(CellState)
+        out.net, out.ncell, out.ld, out.fields = self.net, self.ncell, self.ld, self.fields
+        out.arrays = {k: v.cpu().numpy() for k, v in self.t.items()}
+        return out
+
+    def to_abi(self) -> abi.State:
+        s = abi.State()
+        s.ld = self.ld
+        for name, _ in abi.State._fields_:
+            if name == "ld":
+                continue
+            tt = self.t.get(name)
+            setattr(s, name, None if tt is None or tt.shape[0] == 0 else tt.data_ptr())
+        return s
+
+
+class Context:
+    """pfchem_ctx: network + options uploaded to one device."""
+
+    def __init__(self, net: ReactionNetwork, opt: SolverOptions, device: int = 0):
+        self.L = load()
+        self.net = net
+        self.opt = opt
+        self._n = net.to_abi()
+        self._o = opt.to_abi()
+        h = C.c_void_p()
+        rc = self.L.pfchem_create(C.byref(self._n), C.byref(self._o), device, C.byref(h))
+        if rc:
+            raise PfchemError(rc, "pfchem_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pfchem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise PfchemError(rc, "%s: %s" % (what, self.L.pfchem_last_error(self.h).decode()))
+
+    def step(self, st: DeviceState, dt: float, variant: int = abi.KERNEL_BATCHED_MIXED,
+             stream=None, ncell: Optional[int] = None):
+        s = st.to_abi()
+        sp = None
+        if stream is not None:
+            sp = stream.cuda_stream
+        else:
+            import torch
+            sp = torch.cuda.current_stream().cuda_stream
+        n = st.ncell if ncell is None else ncell
+        self._check(self.L.pfchem_step(self.h, n, C.byref(s), dt, variant, sp), "pfchem_step")
+
+    def step_host(self, st: CellState, dt: float, variant: int = abi.KERNEL_BATCHED_MIXED):
+        s = st.to_abi()
+        rc = self.L.pfchem_step_host(self.h, st.ncell, C.byref(s), dt, variant)
+        if rc and rc != abi.PFCHEM_ERR_CELLS:
+            self._check(rc, "pfchem_step_host")
+        return rc
+
+    def update_activity(self, st: DeviceState, stream=None):
+        import torch
+        s = st.to_abi()
+        sp = (stream or torch.cuda.current_stream()).cuda_stream
+        self._check(self.L.pfchem_update_activity(self.h, st.ncell, C.byref(s), sp),
+                    "pfchem_update_activity")
+
+    def set_launch(self, variant, lanes_per_cell, block_size=256, blocks_per_cu=4):
+        c = abi.Launch(lanes_per_cell, block_size, blocks_per_cu, 0)
+        self._check(self.L.pfchem_set_launch(self.h, variant, C.byref(c)), "pfchem_set_launch")
+
+    def get_launch(self, variant):
+        c = abi.Launch()
+        self._check(self.L.pfchem_get_launch(self.h, variant, C.byref(c)), "pfchem_get_launch")
+        return dict(lanes_per_cell=c.lanes_per_cell, block_size=c.block_size,
+                    blocks_per_cu=c.blocks_per_cu)

@@ -1,0 +1,67 @@
+"""Helpers shared by the GPU parity tests: run a workload through the oracle
+(CPU, the checker) and through the HIP kernels, and compare."""
+import numpy as np
+
+import oracle_lib as O
+from pfchem import abi
+from pfchem.lib import Context, DeviceState
+
+FIELDS_F64 = ("pri_molal", "total", "sec_molal", "immobile", "mnrl_volfrac", "mnrl_rate")
+
+
+def run_oracle(wl, nthreads=0, state=None):
+    st = (state or wl.state).copy()
+    rec = O.Records(st.ncell)
+    nfail = O.step_batch(wl.net, wl.opt, st, wl.dt, nthreads, rec)
+    return st, rec, nfail
+
+
+def run_gpu(wl, variant, lanes=None, block=None, bpc=None, state=None, ctx=None, repeat=1):
+    import torch
+    ctx = ctx or Context(wl.net, wl.opt)
+    if lanes is not None:
+        cur = ctx.get_launch(variant)
+        ctx.set_launch(variant, lanes, block or cur["block_size"], bpc or cur["blocks_per_cu"])
+    host = state or wl.state
+    out = None
+    for _ in range(repeat):
+        ds = DeviceState(host)
+        ctx.step(ds, wl.dt, variant)
+        torch.cuda.synchronize()
+        out = ds.to_host()
+    return out, ctx
+
+
+def rel_err(a, b, floor=0.0):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(b), floor)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(d == 0, 0.0, d / s)
+    return r
+
+
+def compare(ref, got, ncell, fields=FIELDS_F64, floor_frac=1e-12):
+    """max relative error per field over the first ncell cells.  Totals can be
+    near zero, so their relative error uses a floor of floor_frac times the
+    largest magnitude of that species over the cells."""
+    out = {}
+    for f in fields:
+        a = ref[f][:, :ncell]
+        b = got[f][:, :ncell]
+        if a.size == 0:
+            continue
+        floor = floor_frac * np.abs(a).max(axis=1, keepdims=True) if f in ("total", "mnrl_rate") else 0.0
+        out[f] = float(rel_err(b, a, floor).max())
+    return out
+
+
+def gate_rel_residual(wl, gpu_state, rec, cells, ref_state):
+    """||F(x_gpu)||/||F(x0)|| of the oracle's last sub-step system, for cells
+    whose oracle run used exactly one sub-step (same system for both)."""
+    ns = ref_state["num_sub_steps"][0]
+    cells = [c for c in cells if ns[c] == 1]
+    if not cells:
+        return np.zeros(0)
+    return O.rel_residuals(wl.net, wl.opt, gpu_state, rec, cells)
