@@ -1,0 +1,245 @@
+#!/usr/bin/env python3
+"""Reaction-step throughput on one MI355X: cells/s of the operator-split
+chemistry step (RStep over every cell) at relative residual <= 1e-10.
+
+Workload: the random aqueous network with 15 primaries, 60 complexes and
+5 kinetic minerals (0-5 active per cell) on 1,048,576 cells, dt = 3600 s,
+synthetic seeded inputs.  A "step" is one pfchem_step over all cells; each
+timed step works on its own HBM-resident copy of the same input state, so
+all K steps solve the same problem back to back.
+
+Reported: the batched mixed-precision kernel (headline value), the plain
+FP64 one-cell-per-thread kernel it is measured against, the batched FP64
+kernel, the roofline of the dominant kernel, the FP64 CPU oracle timed on
+this host's cores (cpu_baseline), and the parity of the GPU answer against
+that oracle (relative concentration error and FP64 relative residual).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "pflotran-elm-interface_amd"), os.path.join(ROOT, "oracle")]
+
+METRIC = ("reaction-step cells/s @ rel.resid≤1e-10: plain FP64 vs batched "
+          "mixed+adjoint")
+MI355X_FP64_VALU_TFLOPS = 78.6
+MI355X_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=1 << 20)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (calibrates the "
+                         "HBM byte counters of a profiler) and exit")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8}), flush=True)
+        return 0
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the reaction step is cell-local and this build "
+              "runs on one MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+    from pfchem import abi, workloads, flops
+    from pfchem.lib import Context, DeviceState
+
+    t0 = time.time()
+    wl = workloads.random_network_workload(ncell=args.cells)
+    net, opt, host = wl.net, wl.opt, wl.state
+    n = host.ncell
+    log("[bench] workload %s: %d cells generated in %.1f s" % (wl.name, n, time.time() - t0))
+
+    ctx = Context(net, opt)
+    dev0 = DeviceState(host)
+
+    def copies(k):
+        out = []
+        for _ in range(k):
+            d = DeviceState.__new__(DeviceState)
+            d.net, d.ncell, d.ld, d.fields = dev0.net, dev0.ncell, dev0.ld, dev0.fields
+            d.t = {name: t.clone() for name, t in dev0.t.items()}
+            out.append(d)
+        return out
+
+    def run(variant, nsteps, nwarm):
+        work = copies(nwarm + nsteps)
+        for i in range(nwarm):
+            ctx.step(work[i], wl.dt, variant)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        ev0.record()
+        for i in range(nsteps):
+            kev[i][0].record()
+            ctx.step(work[nwarm + i], wl.dt, variant)
+            kev[i][1].record()
+        ev1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        total_ms = ev0.elapsed_time(ev1)
+        per = [a.elapsed_time(b) for a, b in kev]
+        return work[-1], total_ms, per, wall
+
+    # --- plain FP64 one-cell-per-thread kernel (the reference point) ---
+    plain_state, plain_ms, plain_per, _ = run(abi.KERNEL_PLAIN_FP64, args.plain_steps, 1)
+    plain_cps = args.plain_steps * n / (plain_ms * 1e-3)
+    log("[bench] plain_fp64    %.2f ms/step  %.3e cells/s" % (plain_ms / args.plain_steps, plain_cps))
+    # --- batched FP64 ---
+    b64_state, b64_ms, _, _ = run(abi.KERNEL_BATCHED_FP64, max(2, args.steps // 2), 1)
+    b64_cps = max(2, args.steps // 2) * n / (b64_ms * 1e-3)
+    log("[bench] batched_fp64  %.2f ms/step  %.3e cells/s" % (b64_ms / max(2, args.steps // 2), b64_cps))
+    # --- batched mixed precision (headline) ---
+    mix_state, mix_ms, mix_per, wall = run(abi.KERNEL_BATCHED_MIXED, args.steps, args.warmup)
+    ms_per_step = mix_ms / args.steps
+    value = args.steps * n / (mix_ms * 1e-3)
+    log("[bench] batched_mixed %.2f ms/step  %.3e cells/s (wall %.3f s)" % (ms_per_step, value, wall))
+
+    got = mix_state.to_host()
+    status = got["status"][0, :n]
+    iters_gpu = got["num_iterations"][0, :n]
+    launch = ctx.get_launch(abi.KERNEL_BATCHED_MIXED)
+
+    # --- CPU oracle: baseline timing + parity gate + reference iteration counts ---
+    cpu = None
+    parity = {}
+    iters_ref = iters_gpu
+    if not args.no_cpu:
+        import oracle_lib as O
+        # the host share of one GPU: OMP_NUM_THREADS when set, else <= 16 cores
+        nthr = (args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                or min(16, os.cpu_count() or 1))
+        # one-thread sample, bounded to ~a few seconds
+        sample = min(n, 16384)
+        sub = host.subset(np.arange(sample))
+        t1 = time.perf_counter()
+        O.step_batch(net, opt, sub, wl.dt, 1)
+        t_one = time.perf_counter() - t1
+        # all threads, full workload (records for the residual gate)
+        ref = host.copy()
+        rec = O.Records(n)
+        t1 = time.perf_counter()
+        nfail_ref = O.step_batch(net, opt, ref, wl.dt, nthr, rec)
+        t_all = time.perf_counter() - t1
+        cpu = {"value": n / t_all, "unit": "cells/s", "cores": nthr, "kind": "port",
+               "sample": "FP64 C oracle (RStep restatement), all %d cells of the same "
+                         "workload on %d threads (%.2f s); 1 thread on %d cells: %.3e cells/s"
+                         % (n, nthr, t_all, sample, sample / t_one),
+               "one_core_cells_per_s": sample / t_one}
+        iters_ref = ref["num_iterations"][0, :n]
+        fr_ref = O.record_fields(rec, "final_rel_residual")
+        one_sub = ref["num_sub_steps"][0, :n] == 1
+
+        def parity_of(st, label):
+            g = st.to_host()
+            gs = g["status"][0, :n]
+            ok = (gs == 0) & (ref["status"][0, :n] == 0)
+            errs = {}
+            for f in ("pri_molal", "sec_molal", "total", "mnrl_volfrac", "mnrl_rate"):
+                a = ref[f][:, :n][:, ok]
+                b = g[f][:, :n][:, ok]
+                if a.size == 0:
+                    continue
+                # totals and rates can cancel to ~0: relative to 1e-12 of the
+                # largest magnitude of that species
+                floor = (1e-12 * np.abs(a).max(axis=1, keepdims=True)
+                         if f in ("total", "mnrl_rate") else 0.0)
+                d = np.abs(b - a)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    r = np.where(d == 0, 0.0, d / np.maximum(np.abs(a), floor))
+                errs[f] = float(r.max())
+            cells = np.where(ok & one_sub)[0]
+            chk = cells[:: max(1, cells.size // 20000)]
+            rr = O.rel_residuals(net, opt, g, rec, chk) if chk.size else np.zeros(1)
+            floor = np.maximum(1e-10, fr_ref[chk]) if chk.size else np.ones(1)
+            return {label + "_failed_cells": int((gs != 0).sum()),
+                    label + "_max_rel_err": errs,
+                    label + "_max_rel_conc_err": max(errs.get("pri_molal", 0.0),
+                                                     errs.get("sec_molal", 0.0)),
+                    label + "_max_rel_residual": float(rr.max()),
+                    label + "_cells_rel_residual_le_1e-10": float(np.mean(rr <= 1e-10)),
+                    label + "_max_rel_residual_over_oracle_floor": float((rr / floor).max()),
+                    label + "_residual_cells_checked": int(chk.size)}
+
+        parity.update(parity_of(mix_state, "mixed"))
+        parity.update(parity_of(b64_state, "fp64"))
+        parity["oracle_failed_cells"] = int(nfail_ref)
+        parity["oracle_max_rel_residual"] = float(fr_ref[(ref["status"][0, :n] == 0)].max())
+        log("[bench] oracle %.2f s on %d threads; parity: %s" % (t_all, nthr, json.dumps(parity)))
+
+    # --- roofline of the dominant kernel (team_step_kernel, mixed) ---
+    active = host["mnrl_area"][:, :n] > 0
+    f_it = flops.per_iteration_flops(net, opt, active)
+    flop_alg = float((iters_ref * f_it).sum())
+    k_ms = float(np.mean(mix_per))
+    achieved = flop_alg / (k_ms * 1e-3) / 1e12
+    bytes_alg = flops.state_bytes_per_cell(net) * n
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_team_step_random15.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    roofline = {"bound": "valu", "achieved": achieved, "peak": MI355X_FP64_VALU_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_VALU_TFLOPS,
+                "traffic": traffic,
+                "kernel": "team_step_kernel<16,float>",
+                "algorithmic_flop_per_launch": flop_alg,
+                "algorithmic_bytes_per_launch": bytes_alg,
+                "hbm_gbs_algorithmic": bytes_alg / (k_ms * 1e-3) / 1e9,
+                "kernel_ms_mean": k_ms}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 residual/logK/update + f32 Jacobian/LU", "data": "synthetic",
+        "config": {"workload": "random aqueous network: 15 primaries, 60 complexes, "
+                               "5 kinetic minerals (0-5 active per cell)",
+                   "cells": n, "dt_s": wl.dt, "seed": wl.seed,
+                   "kernel": "batched_mixed", "launch": launch,
+                   "tolerances": {"max_rel_residual_tolerance": opt.max_rel_residual_tolerance,
+                                  "max_relative_change_tolerance": opt.max_relative_change_tolerance,
+                                  "max_residual_tolerance": opt.max_residual_tolerance,
+                                  "maximum_reaction_iterations": opt.maximum_reaction_iterations}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_fp64_cells_per_s": plain_cps,
+        "batched_fp64_cells_per_s": b64_cps,
+        "speedup_vs_plain_fp64": value / plain_cps,
+        "failed_cells": int((status != 0).sum()),
+        "newton_iterations_mean": float(iters_gpu.mean()),
+    }
+    out.update(parity)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

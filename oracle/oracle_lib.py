@@ -9,8 +9,7 @@ import subprocess
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
 _lib = None
@@ -61,6 +60,17 @@ class Records:
 
     def at(self, cell):
         return self.ptr + cell * self.size
+
+
+def record_fields(records, name):
+    """numpy view of one double field of every orc_substep_record"""
+    import ctypes as C
+    off = {"dt": 0, "r0_norm": None, "final_rel_residual": None}
+    size = records.size
+    # r0_norm and final_rel_residual are the last two doubles of the record
+    idx = {"r0_norm": size - 16, "final_rel_residual": size - 8, "dt": 0}[name]
+    raw = np.frombuffer(records.buf, dtype=np.uint8, count=size * records.ncell)
+    return np.ndarray((records.ncell,), dtype=np.float64, buffer=raw, offset=idx, strides=(size,))
 
 
 def step_batch(net, opt, state, dt, nthreads=0, records=None):

@@ -353,7 +353,7 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
     e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
     e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cfg, ctx->counter,
-                         ctx->num_cus, s);
+                         ctx->num_cus, s, d.status != nullptr);
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
   return PFCHEM_OK;

@@ -24,10 +24,12 @@ hipError_t launch_activity_lag(const DevNet& n, const DevState& s, int64_t ncell
 // launcher zeroes with a fill kernel (no memset node) before the launch.
 int team_max_ncomp();
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
-// dn / ds: device copies of the network / state descriptors
+// dn / ds: device copies of the network / state descriptors.  The mixed
+// variant runs the FP32 pass and then an FP64 pass over the cells the FP32
+// pass could not converge (needs the per-cell status array).
 hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,
                             int64_t ncell, double dt, bool mixed, const LaunchCfg& cfg,
                             unsigned long long* counter, int num_cus,
-                            hipStream_t stream);
+                            hipStream_t stream, bool has_status);
 
 }  // namespace pfchem

@@ -33,6 +33,8 @@
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
 
+#include <type_traits>
+
 namespace pfchem {
 namespace {
 
@@ -44,31 +46,117 @@ __device__ __forceinline__ void team_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---------------------------------------------------------------------
+// Cross-lane primitives inside a team of L consecutive lanes, built on DPP
+// (quad_perm, row_half_mirror, row_mirror, row_newbcast), ds_swizzle and
+// readlane, so that reductions and broadcasts stay in the VALU / LDS-swizzle
+// path instead of ds_bpermute.  All lanes of a team must be active.
+// ---------------------------------------------------------------------
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for_down(F&& f) {  // E-1 down to B
+  if constexpr (B < E) {
+    f(std::integral_constant<int, E - 1>{});
+    static_for_down<B, E - 1>(f);
+  }
+}
+
+// broadcast lane P of every team to the whole team (32-bit)
+template <int L, int P>
+__device__ __forceinline__ int bcast32(int v) {
+  if constexpr (L == 4) {
+    return __builtin_amdgcn_update_dpp(0, v, P | (P << 2) | (P << 4) | (P << 6), 0xF, 0xF, false);
+  } else if constexpr (L == 8) {
+    return __builtin_amdgcn_ds_swizzle(v, 0x18 | (P << 5));  // BROADCAST in groups of 8
+  } else if constexpr (L == 16) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + P, 0xF, 0xF, false);  // row_newbcast:P
+  } else if constexpr (L == 32) {
+    return __builtin_amdgcn_ds_swizzle(v, (P << 5));  // BROADCAST in groups of 32
+  } else {
+    return __builtin_amdgcn_readlane(v, P);
+  }
+}
+template <int L, int P>
+__device__ __forceinline__ int bcast(int v) {
+  return bcast32<L, P>(v);
+}
+template <int L, int P>
+__device__ __forceinline__ float bcast(float v) {
+  return __int_as_float(bcast32<L, P>(__float_as_int(v)));
+}
+template <int L, int P>
+__device__ __forceinline__ double bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = bcast32<L, P>((int)(b & 0xffffffffll));
+  const int hi = bcast32<L, P>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// value of the partner lane at butterfly level `lev` (0: xor 1, 1: xor 2,
+// 2: mirror within 8, 3: mirror within 16, 4: swap halves of 32)
+template <int LEV>
+__device__ __forceinline__ int partner32(int v) {
+  if constexpr (LEV == 0) return __builtin_amdgcn_update_dpp(0, v, 1 | (0 << 2) | (3 << 4) | (2 << 6), 0xF, 0xF, false);
+  else if constexpr (LEV == 1) return __builtin_amdgcn_update_dpp(0, v, 2 | (3 << 2) | (0 << 4) | (1 << 6), 0xF, 0xF, false);
+  else if constexpr (LEV == 2) return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
+  else if constexpr (LEV == 3) return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
+  else return __builtin_amdgcn_ds_swizzle(v, 0x1F | (0x10 << 10));  // SWAP 16
+}
+template <int LEV>
+__device__ __forceinline__ double partner(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = partner32<LEV>((int)(b & 0xffffffffll));
+  const int hi = partner32<LEV>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int LEV>
+__device__ __forceinline__ float partner(float v) {
+  return __int_as_float(partner32<LEV>(__float_as_int(v)));
+}
+template <int LEV>
+__device__ __forceinline__ int partner(int v) {
+  return partner32<LEV>(v);
+}
+
+// all-reduce over the team; every lane gets the bit-identical result
+// (each level combines a lane with a distinct partner, and the combining
+// operations used here are commutative)
+template <int L, typename T, typename Op>
+__device__ __forceinline__ T team_reduce(T v, Op op) {
+  static_assert(L == 4 || L == 8 || L == 16 || L == 32 || L == 64, "team width");
+  v = op(v, partner<0>(v));
+  v = op(v, partner<1>(v));
+  if constexpr (L >= 8) v = op(v, partner<2>(v));
+  if constexpr (L >= 16) v = op(v, partner<3>(v));
+  if constexpr (L >= 32) v = op(v, partner<4>(v));
+  if constexpr (L == 64) {
+    const T a = bcast<64, 0>(v), b = bcast<64, 32>(v);
+    v = op(a, b);
+  }
+  return v;
+}
 template <int L>
 __device__ __forceinline__ double team_sum(double v) {
-#pragma unroll
-  for (int o = L / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, L);
-  return v;
+  return team_reduce<L>(v, [](double a, double b) { return a + b; });
 }
 template <int L>
 __device__ __forceinline__ double team_max(double v) {
-#pragma unroll
-  for (int o = L / 2; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, L));
-  return v;
+  return team_reduce<L>(v, [](double a, double b) { return fmax(a, b); });
 }
 template <int L>
 __device__ __forceinline__ double team_min(double v) {
-#pragma unroll
-  for (int o = L / 2; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, L));
-  return v;
+  return team_reduce<L>(v, [](double a, double b) { return fmin(a, b); });
 }
 template <int L>
 __device__ __forceinline__ int team_or(int v) {
-#pragma unroll
-  for (int o = L / 2; o >= 1; o >>= 1) v |= __shfl_xor(v, o, L);
-  return v;
+  return team_reduce<L>(v, [](int a, int b) { return a | b; });
 }
-
 // LDS tile of one team (byte offsets); same formula on host and device
 struct TeamLds {
   int lnact, rs, tot, racc, xv, mvf, marea, mrate, mk, lng, J, prow, bytes;
@@ -162,12 +250,18 @@ __device__ __forceinline__ MineralRate mineral_rate(const DevNet& n, int m, doub
   return r;
 }
 
+// relative FP64 residual a mixed-precision iterate must reach before a
+// small Newton step may end the iteration
+constexpr double kMixedStepExitResidual = 1.0e-11;
+// transient per-cell status: the FP32 pass hands the cell to the FP64 pass
+constexpr int kEscalate = 16;
+
 enum Outcome : int { kContinue = 0, kConverged = 1, kFailMaxIt = 2, kFailSingular = 3 };
 
 template <int L, typename JT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
-                 unsigned long long* __restrict__ counter) {
+                 unsigned long long* __restrict__ counter, int rescue_only) {
   const DevNet& n = *np;
   const DevState& s = *sp;
   extern __shared__ __align__(16) char smem[];
@@ -192,6 +286,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 
   const int naq = n.naq, nimm = n.nimm, N = n.ncomp, nc = n.nc, nm = n.nm;
   const bool mine = t < N;              // this lane owns a species row
+  const bool escalate = s.status != nullptr;  // FP32 failures go to the FP64 pass
   const bool is_aq = t < naq;
   const int64_t ld = s.ld;
 
@@ -245,6 +340,8 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     if (cell < 0 && !exhausted && got >= 0) {
       if (got >= ncell) {
         exhausted = true;
+      } else if (rescue_only && s.status[got] != kEscalate) {
+        // FP64 rescue pass: only cells the FP32 pass handed over
       } else {
         cell = got;
         const int64_t c = cell;
@@ -433,10 +530,34 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     }
     team_sync();
     // phase D: residual, totals, accumulation diagonal, decay
+    // Row t of the scaled log-form Jacobian: the LDS tile plus the
+    // accumulation diagonal (1 after scaling) and first-order decay, the
+    // identity for a dry cell; columns divided by x_j for the linear form.
+    auto build_row = [&](JT* a) {
+#pragma unroll
+      for (int j = 0; j < L; ++j) a[j] = mine ? Jt[t * (L + 1) + j] : (JT)0;
+      if (!mine) return;
+      if (wet) {
+        double d = 1.0;
+        if (!is_aq)
+          for (int r = 0; r < n.ndecay; ++r)
+            if (n.decay_id[r] == t - naq) d += n.decay_k[r] * vol * x * rs;
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+          if (j == t) a[j] += (JT)d;
+      } else {
+#pragma unroll
+        for (int j = 0; j < L; ++j) a[j] = (JT)(j == t ? 1 : 0);
+      }
+      if (!n.log_form) {
+#pragma unroll
+        for (int j = 0; j < L; ++j)
+          if (j < N) a[j] = (JT)((double)a[j] / xv[j]);
+      }
+    };
     double mx = 0.0, nr2 = 0.0;
     JT arow[L];
-#pragma unroll
-    for (int j = 0; j < L; ++j) arow[j] = (JT)0;
+    build_row(arow);
     if (mine) {
       if (is_aq) {
         total = (x + tot[t]) * dkl;
@@ -444,30 +565,11 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       } else {
         total = x;
         res = wet ? (x * vol - fixed) / tdt : 0.0;
+        if (wet)
+          for (int r = 0; r < n.ndecay; ++r)
+            if (n.decay_id[r] == t - naq) res += n.decay_k[r] * vol * x;
       }
       if (wet) res += racc[t];
-#pragma unroll
-      for (int j = 0; j < L; ++j) arow[j] = Jt[t * (L + 1) + j];
-      if (wet) {
-        // accumulation diagonal (scaled to 1) and first-order decay
-#pragma unroll
-        for (int j = 0; j < L; ++j)
-          if (j == t) arow[j] += (JT)1;
-        if (!is_aq) {
-          for (int r = 0; r < n.ndecay; ++r) {
-            if (n.decay_id[r] == t - naq) {
-              const double kc = n.decay_k[r] * vol;
-              res += kc * x;
-#pragma unroll
-              for (int j = 0; j < L; ++j)
-                if (j == t) arow[j] += (JT)(kc * x * rs);
-            }
-          }
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < L; ++j) arow[j] = (JT)(j == t ? 1 : 0);
-      }
       mx = fabs(res);
       nr2 = res * res;
     }
@@ -481,12 +583,6 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       outcome = kConverged;
     } else {
       // ---------------- linear solve (RSolve) ----------------
-      if (!n.log_form) {
-        // linear formulation: columns back to d res / d m_j
-#pragma unroll
-        for (int j = 0; j < L; ++j)
-          if (j < N) arow[j] = (JT)((double)arow[j] / xv[j]);
-      }
       double amax = 0.0;
 #pragma unroll
       for (int j = 0; j < L; ++j) amax = fmax(amax, (double)fabs(arow[j]));
@@ -494,61 +590,112 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       if (zero_row) {
         outcome = kFailSingular;
       } else {
-        const JT vv = mine ? (JT)(1.0 / amax) : (JT)0;
-        JT rhs = mine ? (JT)(res * rs) : (JT)0;
-        bool used = !mine;
-        int mystep = -1;
-#pragma unroll
-        for (int p = 0; p < L; ++p) {
-          if (p < N) {
-            // scaled partial pivoting; ties go to the larger row index
-            JT cand = used ? (JT)-1 : (JT)fabs(arow[p]) * vv;
-            int ci = t;
-#pragma unroll
-            for (int o = L / 2; o >= 1; o >>= 1) {
-              const JT oc = __shfl_xor(cand, o, L);
-              const int oi = __shfl_xor(ci, o, L);
-              if (oc > cand || (oc == cand && oi > ci)) {
-                cand = oc;
-                ci = oi;
-              }
-            }
-            const int piv = ci;
-            if (t == piv) {
-              if (arow[p] == (JT)0) arow[p] = (JT)1.0e-20;
-#pragma unroll
-              for (int j = p; j < L; ++j) prow[j] = arow[j];
-              prow[L] = rhs;
-              used = true;
-              mystep = p;
-            }
-            team_sync();
-            if (!used) {
-              const JT l = arow[p] / prow[p];
-              arow[p] = l;
-#pragma unroll
-              for (int j = p + 1; j < L; ++j) arow[j] -= l * prow[j];
-              rhs -= l * prow[L];
-            }
-            team_sync();
-          }
-        }
-        // back substitution: x_p from the row pivoted at step p
+        const JT rhs0 = mine ? (JT)(res * rs) : (JT)0;
+        JT rhs = rhs0;
         JT xs[L];
-#pragma unroll
-        for (int p = L - 1; p >= 0; --p) {
-          xs[p] = (JT)0;
+        // Fixed-order elimination: the pivot of step p is row p.  The
+        // log-form Jacobian of this system (accumulation S^T M S + diag(x),
+        // TST minerals c nu nu^T with c >= 0) is symmetric positive definite
+        // up to the row scaling, so no pivot search is needed; a pivot that
+        // fails the guard sends the team to the pivoting path below.
+        constexpr double kPivTol = sizeof(JT) == 4 ? 1.0e-6 : 1.0e-12;
+        int bad = 0;
+        static_for<0, L>([&](auto P) {
+          constexpr int p = decltype(P)::value;
           if (p < N) {
-            if (mystep == p) {
-              JT sum = rhs;
-#pragma unroll
-              for (int j = p + 1; j < L; ++j) sum -= arow[j] * xs[j];
-              prow[0] = sum / arow[p];
+            const JT pv = bcast<L, p>(arow[p]);
+            if (t == p && !((double)fabs(arow[p]) > kPivTol * amax)) bad = 1;
+            JT pr[L];
+            static_for<p + 1, L>([&](auto Q) {
+              constexpr int j = decltype(Q)::value;
+              pr[j] = bcast<L, p>(arow[j]);
+            });
+            const JT prhs = bcast<L, p>(rhs);
+            if (t > p) {
+              const JT l = arow[p] / pv;
+              arow[p] = l;
+              static_for<p + 1, L>([&](auto Q) {
+                constexpr int j = decltype(Q)::value;
+                arow[j] -= l * pr[j];
+              });
+              rhs -= l * prhs;
             }
-            team_sync();
-            xs[p] = prow[0];
-            team_sync();
-            if (t == p) delta = (double)xs[p];
+          }
+        });
+        bad = team_or<L>(bad);
+        if (!bad) {
+          // back substitution, rows in natural order
+          static_for_down<0, L>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            xs[p] = (JT)0;
+            if (p < N) {
+              JT sum = rhs;
+              static_for<p + 1, L>([&](auto Q) {
+                constexpr int j = decltype(Q)::value;
+                sum -= arow[j] * xs[j];
+              });
+              xs[p] = bcast<L, p>(sum / arow[p]);
+              if (t == p) delta = (double)xs[p];
+            }
+          });
+        } else {
+          // scaled partial pivoting (LUDecomposition1, utility.F90:597-688)
+          build_row(arow);
+          rhs = rhs0;
+          const JT vv = mine ? (JT)(1.0 / amax) : (JT)0;
+          bool used = !mine;
+          int mystep = -1;
+#pragma unroll
+          for (int p = 0; p < L; ++p) {
+            if (p < N) {
+              // ties go to the larger row index, as in the Crout loop
+              JT cand = used ? (JT)-1 : (JT)fabs(arow[p]) * vv;
+              int ci = t;
+#pragma unroll
+              for (int o = L / 2; o >= 1; o >>= 1) {
+                const JT oc = __shfl_xor(cand, o, L);
+                const int oi = __shfl_xor(ci, o, L);
+                if (oc > cand || (oc == cand && oi > ci)) {
+                  cand = oc;
+                  ci = oi;
+                }
+              }
+              const int piv = ci;
+              if (t == piv) {
+                if (arow[p] == (JT)0) arow[p] = (JT)1.0e-20;
+#pragma unroll
+                for (int j = p; j < L; ++j) prow[j] = arow[j];
+                prow[L] = rhs;
+                used = true;
+                mystep = p;
+              }
+              team_sync();
+              if (!used) {
+                const JT l = arow[p] / prow[p];
+                arow[p] = l;
+#pragma unroll
+                for (int j = p + 1; j < L; ++j) arow[j] -= l * prow[j];
+                rhs -= l * prow[L];
+              }
+              team_sync();
+            }
+          }
+          // back substitution: x_p from the row pivoted at step p
+#pragma unroll
+          for (int p = L - 1; p >= 0; --p) {
+            xs[p] = (JT)0;
+            if (p < N) {
+              if (mystep == p) {
+                JT sum = rhs;
+#pragma unroll
+                for (int j = p + 1; j < L; ++j) sum -= arow[j] * xs[j];
+                prow[0] = sum / arow[p];
+              }
+              team_sync();
+              xs[p] = prow[0];
+              team_sync();
+              if (t == p) delta = (double)xs[p];
+            }
           }
         }
         // ---------------- update (reaction.F90:3977-4049) ----------------
@@ -569,7 +716,11 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           if (!(rc == rc)) rc = -1.0;
         }
         const double mrc = team_max<L>(rc);
-        if (mrc >= 0.0 && mrc < n.tol_change) {
+        // With an FP32 factorization the step is only as accurate as the
+        // factorization, so a small-step exit is accepted only when the FP64
+        // residual of the iterate confirms it.
+        const bool step_exit_ok = sizeof(JT) == 8 || nr <= kMixedStepExitResidual * r0;
+        if (mrc >= 0.0 && mrc < n.tol_change && step_exit_ok) {
           outcome = kConverged;
         } else if (mine) {
           x = latest;
@@ -600,6 +751,12 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         finish = true;
         if (small) total = small_val;
       }
+    } else if (outcome != kContinue && sizeof(JT) == 4 && escalate) {
+      // The FP32 factorization could not bring this cell to convergence:
+      // leave its inputs untouched and hand it to the FP64 pass.
+      if (t == 0 && s.status) s.status[c] = kEscalate;
+      cell = -1;
+      continue;
     } else if (outcome != kContinue) {
       nit_total += nit;
       if (outcome == kFailMaxIt) total = init_tot;  // totals / immobile restored
@@ -652,7 +809,7 @@ __global__ void zero_counter(unsigned long long* c) {
 template <int L, typename JT>
 hipError_t launch_L(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                     double dt, const LaunchCfg& cfg, unsigned long long* counter,
-                    int num_cus, hipStream_t stream) {
+                    int num_cus, hipStream_t stream, int rescue_only) {
   const TeamLds lay = team_lds_layout(L, n.nc, (int)sizeof(JT));
   const int block = cfg.block_size;
   const size_t lds = (size_t)(block / L) * lay.bytes;
@@ -662,8 +819,8 @@ hipError_t launch_L(const DevNet& n, const DevNet* dn, const DevState* ds, int64
   if (blocks > max_blocks) blocks = max_blocks;
   if (blocks < 1) blocks = 1;
   zero_counter<<<1, 64, 0, stream>>>(counter);
-  team_step_kernel<L, JT><<<dim3((unsigned)blocks), dim3(block), lds, stream>>>(dn, ds, ncell, dt,
-                                                                               counter);
+  team_step_kernel<L, JT><<<dim3((unsigned)blocks), dim3(block), lds, stream>>>(
+      dn, ds, ncell, dt, counter, rescue_only);
   return hipGetLastError();
 }
 
@@ -686,12 +843,19 @@ int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed) {
 
 hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,
                             int64_t ncell, double dt, bool mixed, const LaunchCfg& cfg,
-                            unsigned long long* counter, int num_cus, hipStream_t stream) {
+                            unsigned long long* counter, int num_cus, hipStream_t stream,
+                            bool has_status) {
   switch (cfg.lanes_per_cell) {
 #define PFCHEM_CASE(LL)                                                                   \
   case LL:                                                                                \
-    return mixed ? launch_L<LL, float>(n, dn, ds, ncell, dt, cfg, counter, num_cus, stream) \
-                 : launch_L<LL, double>(n, dn, ds, ncell, dt, cfg, counter, num_cus, stream);
+    if (!mixed) return launch_L<LL, double>(n, dn, ds, ncell, dt, cfg, counter, num_cus, stream, 0); \
+    {                                                                                     \
+      hipError_t e = launch_L<LL, float>(n, dn, ds, ncell, dt, cfg, counter, num_cus, stream, 0); \
+      if (e != hipSuccess || !has_status) return e;                                       \
+      LaunchCfg c64 = cfg;                                                                \
+      while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;   \
+      return launch_L<LL, double>(n, dn, ds, ncell, dt, c64, counter, num_cus, stream, 1); \
+    }
     PFCHEM_CASE(4)
     PFCHEM_CASE(8)
     PFCHEM_CASE(16)

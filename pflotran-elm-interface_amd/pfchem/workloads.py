@@ -71,7 +71,7 @@ def bench_options(**kw) -> SolverOptions:
         max_dlnC_rreact=5.0,
         max_residual_tolerance=1.0e-30,
         max_rel_residual_tolerance=1.0e-12,
-        max_relative_change_tolerance=1.0e-12,
+        max_relative_change_tolerance=1.0e-13,
     )
     for k, v in kw.items():
         setattr(o, k, v)

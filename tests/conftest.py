@@ -3,7 +3,7 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "pflotran-elm-interface_amd")
-for p in (PKG, os.path.join(ROOT, "tests"), ROOT):
+for p in (PKG, os.path.join(ROOT, "tests"), os.path.join(ROOT, "oracle"), ROOT):
     if p not in sys.path:
         sys.path.insert(0, p)
 

@@ -15,7 +15,8 @@ import gpu_util as G
 pytestmark = pytest.mark.gpu
 
 CONC_TOL = 1e-8      # relative concentration error vs the FP64 oracle
-RESID_TOL = 1e-10    # FP64 relative residual of the kernel's answer
+RESID_TOL = 1e-10    # FP64 relative residual of the kernel's answer (or the
+                     # oracle's own, where rounding keeps it above 1e-10)
 
 VARIANTS = [abi.KERNEL_PLAIN_FP64, abi.KERNEL_BATCHED_FP64, abi.KERNEL_BATCHED_MIXED]
 
@@ -44,10 +45,11 @@ def _check(wl, ref, rec, got, label):
     print(label, errs, flush=True)
     for f, e in errs.items():
         assert e <= CONC_TOL, "%s: %s rel err %.3e > %.1e" % (label, f, e, CONC_TOL)
-    rr = G.gate_rel_residual(wl, got, rec, range(n), ref)
+    rr, bar = G.gate_rel_residual(wl, got, rec, range(n), ref)
     assert rr.size > 0.9 * n
-    print(label, "max rel residual %.3e over %d cells" % (rr.max(), rr.size), flush=True)
-    assert rr.max() <= RESID_TOL
+    print(label, "max rel residual %.3e over %d cells (bar max %.3e)" % (rr.max(), rr.size, bar.max()),
+          flush=True)
+    assert np.all(rr <= bar)
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
