@@ -303,13 +303,16 @@ int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                     const double* fixed_total, const double* fixed_immobile,
                     double dt, double* res, void* stream);
 
-/* Launch configuration chosen for the batched kernels, per size bucket;
-   set by the autotuner (lanes per cell, workgroup size, waves per CU). */
+/* Launch configuration of the batched kernels, set by the autotuner: a
+   team of lanes_per_cell lanes owns one cell (64 / lanes_per_cell cells per
+   wavefront) and each lane owns rows_per_lane rows of the cell's system
+   (lanes_per_cell * rows_per_lane >= ncomp).  Shapes compiled in:
+   1x4 4x1 1x8 2x4 8x1 2x8 4x4 8x2 16x1 8x4 16x2 32x1. */
 typedef struct pfchem_launch {
-  int32_t lanes_per_cell;   /* 1 (cell per lane) or 8/16/32/64 (team)     */
-  int32_t block_size;       /* threads per workgroup (multiple of 64)     */
+  int32_t lanes_per_cell;   /* 1, 2, 4, 8, 16 or 32                       */
+  int32_t block_size;       /* threads per workgroup (64, 128 or 256)     */
   int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CUs      */
-  int32_t pad0;
+  int32_t rows_per_lane;    /* 1, 2, 4 or 8                               */
 } pfchem_launch;
 int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg);
 int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg);

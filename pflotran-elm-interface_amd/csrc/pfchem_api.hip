@@ -18,12 +18,14 @@ struct pfchem_ctx {
   int num_cus = 256;
   DevNet dnet{};
   void* dbuf = nullptr;  // device copy of the network arrays
-  unsigned long long* counter = nullptr;  // work cursor of the persistent kernels
+  unsigned long long* counter = nullptr;  // per-XCD work cursors (128-byte stride)
+  int nxcd = 8;                           // XCDs of the device
+  int max_lds = 65536;                    // LDS bytes per workgroup
   DevNet* dnet_dev = nullptr;             // device copy of dnet
   DevState* dstate_ring = nullptr;        // device slots for per-launch state descriptors
   DevState* hstate_ring = nullptr;        // pinned host mirror the slots are copied from
   int ring_next = 0;
-  LaunchCfg launch[3] = {{1, 128, 0}, {16, 256, 4}, {16, 256, 4}};
+  LaunchCfg launch[3] = {{1, 1, 128, 0}, {16, 1, 256, 4}, {16, 1, 256, 4}};
   pfchem_network net{};   // sizes only (host pointers are not kept)
   pfchem_options opt{};
   bool has_somdec = false;
@@ -109,7 +111,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     delete ctx;
     return PFCHEM_ERR_INVALID;
   }
-  if (naq + nimm > 64 || net->nkinmnrl > 16 || net->neqcplx > 4096) {
+  if (naq + nimm > 32 || net->nkinmnrl > 16 || net->neqcplx > 4096) {
     delete ctx;
     return PFCHEM_ERR_UNSUPPORTED;
   }
@@ -153,6 +155,12 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
     ctx->num_cus = cus;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+          hipSuccess && lds >= 65536)
+    ctx->max_lds = lds;
+  // MI355X / MI300-class parts have 32 CUs per XCD
+  ctx->nxcd = ctx->num_cus >= 64 && ctx->num_cus % 32 == 0 ? ctx->num_cus / 32 : 1;
 
   const int nc = net->neqcplx, nm = net->nkinmnrl;
   Packer P;
@@ -181,7 +189,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     return PFCHEM_ERR_NOMEM;
   }
   e = hipMemcpy(ctx->dbuf, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&ctx->counter, 256);
+  if (e == hipSuccess) e = hipMalloc(&ctx->counter, 128 * 64);
   if (e != hipSuccess) {
     (void)hipFree(ctx->dbuf);
     if (ctx->counter) (void)hipFree(ctx->counter);
@@ -256,11 +264,21 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   ctx->net = *net;
   ctx->opt = *opt;
   ctx->has_somdec = net->somdec != nullptr;
-  // default team width: next power of two >= ncomp (>= 4)
-  int L = 4;
-  while (L < d.ncomp) L *= 2;
+  // default team shape (lanes per cell x rows per lane) by system size;
+  // the autotuner replaces it with a measured choice
+  int L = 16, R = 2;
+  if (d.ncomp <= 4) {
+    L = 4;
+    R = 1;
+  } else if (d.ncomp <= 8) {
+    L = 8;
+    R = 1;
+  } else if (d.ncomp <= 16) {
+    L = 16;
+    R = 1;
+  }
   for (int v = PFCHEM_KERNEL_BATCHED_FP64; v <= PFCHEM_KERNEL_BATCHED_MIXED; ++v) {
-    LaunchCfg c{L, 256, 4};
+    LaunchCfg c{L, R, 256, 4, ctx->max_lds};
     while (c.block_size > 64 && !team_supports(d, c, v == PFCHEM_KERNEL_BATCHED_MIXED))
       c.block_size /= 2;
     ctx->launch[v] = c;
@@ -353,7 +371,7 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
     e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
     e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cfg, ctx->counter,
-                         ctx->num_cus, s, d.status != nullptr);
+                         ctx->num_cus, ctx->nxcd, s, d.status != nullptr);
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
   return PFCHEM_OK;
@@ -387,7 +405,8 @@ int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg) {
   if (cfg->block_size <= 0 || cfg->block_size % 64 || cfg->block_size > 1024 ||
       cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64)
     return fail(ctx, PFCHEM_ERR_INVALID, "bad launch shape");
-  LaunchCfg c{cfg->lanes_per_cell, cfg->block_size, cfg->blocks_per_cu};
+  LaunchCfg c{cfg->lanes_per_cell, cfg->rows_per_lane > 0 ? cfg->rows_per_lane : 1,
+              cfg->block_size, cfg->blocks_per_cu, ctx->max_lds};
   if (variant != PFCHEM_KERNEL_PLAIN_FP64 &&
       !team_supports(ctx->dnet, c, variant == PFCHEM_KERNEL_BATCHED_MIXED))
     return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
@@ -400,7 +419,7 @@ int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg) {
   cfg->lanes_per_cell = ctx->launch[variant].lanes_per_cell;
   cfg->block_size = ctx->launch[variant].block_size;
   cfg->blocks_per_cu = ctx->launch[variant].blocks_per_cu;
-  cfg->pad0 = 0;
+  cfg->rows_per_lane = ctx->launch[variant].rows_per_lane;
   return PFCHEM_OK;
 }
 

@@ -10,8 +10,10 @@ namespace pfchem {
 // persistent-kernel launch shape (see pfchem_launch in include/pfchem.h)
 struct LaunchCfg {
   int lanes_per_cell;
+  int rows_per_lane;
   int block_size;
   int blocks_per_cu;
+  int lds_limit = 65536;  // LDS bytes one workgroup may allocate on the device
 };
 
 int plain_max_ncomp();
@@ -20,8 +22,9 @@ hipError_t launch_plain_step(const DevNet& n, const DevState& s, int64_t ncell,
 hipError_t launch_activity_lag(const DevNet& n, const DevState& s, int64_t ncell,
                                hipStream_t stream);
 
-// batched (team) kernel: `counter` is a device int64 work cursor that the
-// launcher zeroes with a fill kernel (no memset node) before the launch.
+// batched (team) kernel: `cursors` holds one 64-bit work cursor per XCD
+// (128-byte stride) that the launcher zeroes with a fill kernel (no memset
+// node) before each launch.
 int team_max_ncomp();
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
 // dn / ds: device copies of the network / state descriptors.  The mixed
@@ -29,7 +32,7 @@ int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
 // pass could not converge (needs the per-cell status array).
 hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,
                             int64_t ncell, double dt, bool mixed, const LaunchCfg& cfg,
-                            unsigned long long* counter, int num_cus,
+                            unsigned long long* cursors, int num_cus, int nxcd,
                             hipStream_t stream, bool has_status);
 
 }  // namespace pfchem

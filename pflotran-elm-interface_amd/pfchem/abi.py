@@ -193,5 +193,10 @@ class Launch(C.Structure):
         ("lanes_per_cell", C.c_int32),
         ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32),
-        ("pad0", C.c_int32),
+        ("rows_per_lane", C.c_int32),
     ]
+
+
+# (lanes per cell, rows per lane) team shapes compiled into libpfchem.so
+TEAM_SHAPES = [(1, 4), (4, 1), (1, 8), (2, 4), (8, 1), (2, 8), (4, 4), (8, 2), (16, 1),
+               (8, 4), (16, 2), (32, 1)]

@@ -74,6 +74,13 @@ class DeviceState:
         for name, a in host.arrays.items():
             self.t[name] = torch.from_numpy(np.ascontiguousarray(a)).to(device)
 
+    def clone(self) -> "DeviceState":
+        """device-side copy (same shapes, fresh allocations)"""
+        d = DeviceState.__new__(DeviceState)
+        d.net, d.ncell, d.ld, d.fields = self.net, self.ncell, self.ld, self.fields
+        d.t = {name: t.clone() for name, t in self.t.items()}
+        return d
+
     def to_host(self) -> CellState:
         out = CellState.__new__(CellState)
         out.net, out.ncell, out.ld, out.fields = self.net, self.ncell, self.ld, self.fields
@@ -147,12 +154,20 @@ class Context:
         self._check(self.L.pfchem_update_activity(self.h, st.ncell, C.byref(s), sp),
                     "pfchem_update_activity")
 
-    def set_launch(self, variant, lanes_per_cell, block_size=256, blocks_per_cu=4):
-        c = abi.Launch(lanes_per_cell, block_size, blocks_per_cu, 0)
+    def set_launch(self, variant, lanes_per_cell, block_size=256, blocks_per_cu=4,
+                   rows_per_lane=1):
+        c = abi.Launch(lanes_per_cell, block_size, blocks_per_cu, rows_per_lane)
         self._check(self.L.pfchem_set_launch(self.h, variant, C.byref(c)), "pfchem_set_launch")
+
+    def try_set_launch(self, variant, lanes_per_cell, block_size=256, blocks_per_cu=4,
+                       rows_per_lane=1):
+        """set_launch that returns False instead of raising when the shape does
+        not fit this network"""
+        c = abi.Launch(lanes_per_cell, block_size, blocks_per_cu, rows_per_lane)
+        return self.L.pfchem_set_launch(self.h, variant, C.byref(c)) == abi.PFCHEM_OK
 
     def get_launch(self, variant):
         c = abi.Launch()
         self._check(self.L.pfchem_get_launch(self.h, variant, C.byref(c)), "pfchem_get_launch")
-        return dict(lanes_per_cell=c.lanes_per_cell, block_size=c.block_size,
-                    blocks_per_cu=c.blocks_per_cu)
+        return dict(lanes_per_cell=c.lanes_per_cell, rows_per_lane=c.rows_per_lane,
+                    block_size=c.block_size, blocks_per_cu=c.blocks_per_cu)

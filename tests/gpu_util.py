@@ -16,12 +16,14 @@ def run_oracle(wl, nthreads=0, state=None):
     return st, rec, nfail
 
 
-def run_gpu(wl, variant, lanes=None, block=None, bpc=None, state=None, ctx=None, repeat=1):
+def run_gpu(wl, variant, lanes=None, block=None, bpc=None, state=None, ctx=None, repeat=1,
+            rows=None):
     import torch
     ctx = ctx or Context(wl.net, wl.opt)
     if lanes is not None:
         cur = ctx.get_launch(variant)
-        ctx.set_launch(variant, lanes, block or cur["block_size"], bpc or cur["blocks_per_cu"])
+        ctx.set_launch(variant, lanes, block or cur["block_size"], bpc or cur["blocks_per_cu"],
+                       rows or 1)
     host = state or wl.state
     out = None
     for _ in range(repeat):

@@ -71,13 +71,47 @@ def test_random_network_parity(randnet, variant):
     _check(wl, ref, rec, got, "random15/" + abi.KERNEL_NAMES[variant])
 
 
-@pytest.mark.parametrize("lanes,block", [(16, 64), (16, 256), (32, 128), (32, 64)])
-def test_launch_shapes_agree(randnet, lanes, block):
-    """The autotuner's choices (lanes per cell, workgroup size) must not change
-    the answer beyond the parity bar."""
+def _run_shape(wl, variant, lanes, rows, block):
+    """run with the requested team shape, halving the workgroup size until
+    the team tiles fit the device's LDS; None when no size fits"""
+    from pfchem.lib import Context
+    ctx = Context(wl.net, wl.opt)
+    while block >= 64 and not ctx.try_set_launch(variant, lanes, block, 4, rows):
+        block //= 2
+    if block < 64:
+        return None, None
+    got, _ = G.run_gpu(wl, variant, ctx=ctx)
+    return got, block
+
+
+@pytest.mark.parametrize("lanes,rows,block", [(16, 1, 64), (16, 1, 256), (32, 1, 128), (8, 2, 128),
+                                             (4, 4, 128), (2, 8, 64), (8, 4, 64), (16, 2, 128)])
+def test_launch_shapes_agree(randnet, lanes, rows, block):
+    """The autotuner's choices (lanes per cell, rows per lane, workgroup size)
+    must not change the answer beyond the parity bar."""
     wl, ref, rec = randnet
-    got, _ = G.run_gpu(wl, abi.KERNEL_BATCHED_MIXED, lanes=lanes, block=block)
-    _check(wl, ref, rec, got, "random15/mixed/L%d/B%d" % (lanes, block))
+    ran = 0
+    for variant in (abi.KERNEL_BATCHED_MIXED, abi.KERNEL_BATCHED_FP64):
+        got, b = _run_shape(wl, variant, lanes, rows, block)
+        if got is None:
+            continue
+        ran += 1
+        _check(wl, ref, rec, got, "random15/%s/L%dR%d/B%d" % (abi.KERNEL_NAMES[variant], lanes,
+                                                              rows, b))
+    if not ran:
+        pytest.skip("team tiles of L%dR%d exceed the device's LDS" % (lanes, rows))
+
+
+@pytest.mark.parametrize("lanes,rows", [(1, 4), (4, 1), (2, 4), (8, 1), (1, 8)])
+def test_small_team_shapes(calcite, lanes, rows):
+    """Teams smaller than a quad and one-lane teams (a whole cell per lane)
+    on the three-component calcite system."""
+    wl, ref, rec = calcite
+    for variant in (abi.KERNEL_BATCHED_MIXED, abi.KERNEL_BATCHED_FP64):
+        got, b = _run_shape(wl, variant, lanes, rows, 128)
+        assert got is not None
+        _check(wl, ref, rec, got, "calcite/%s/L%dR%d/B%d" % (abi.KERNEL_NAMES[variant], lanes,
+                                                             rows, b))
 
 
 def test_deterministic_and_order_independent(randnet):
