@@ -14,6 +14,39 @@ ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 
 _lib = None
 
+MAXCOMP, MAXCPLX, MAXMNRL, MAXRXN, SOMDEC_MAX_DOWN = 96, 512, 32, 32, 4
+
+
+class OrcCell(C.Structure):
+    """ctypes mirror of orc_cell (oracle/pfchem_oracle.h)"""
+    _fields_ = [
+        ("pri_molal", C.c_double * MAXCOMP),
+        ("total", C.c_double * MAXCOMP),
+        ("immobile", C.c_double * MAXCOMP),
+        ("sec_molal", C.c_double * MAXCPLX),
+        ("pri_act_coef", C.c_double * MAXCOMP),
+        ("sec_act_coef", C.c_double * MAXCPLX),
+        ("ln_act_h2o", C.c_double),
+        ("mnrl_volfrac", C.c_double * MAXMNRL),
+        ("mnrl_area", C.c_double * MAXMNRL),
+        ("mnrl_rate", C.c_double * MAXMNRL),
+        ("porosity", C.c_double), ("sat", C.c_double), ("den_kg", C.c_double),
+        ("volume", C.c_double), ("temp", C.c_double),
+        ("f_t", C.c_double), ("f_w", C.c_double),
+        ("has_f_t", C.c_int), ("has_f_w", C.c_int),
+        ("dtotal", C.c_double * (MAXCOMP * MAXCOMP)),
+        ("eqcplx_logK", C.c_double * MAXCPLX),
+        ("kinmnrl_logK", C.c_double * MAXMNRL),
+        ("sd_upstream_nc", C.c_double * MAXRXN),
+        ("sd_downstream_nc", C.c_double * (MAXRXN * SOMDEC_MAX_DOWN)),
+        ("sd_mineral_c_stoich", C.c_double * MAXRXN),
+        ("sd_mineral_n_stoich", C.c_double * MAXRXN),
+        ("activity_error", C.c_int),
+    ]
+
+    def arr(self, name, n):
+        return np.ctypeslib.as_array(getattr(self, name))[:n]
+
 
 def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
@@ -35,6 +68,27 @@ def lib():
         L.orc_record_rel_residual.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_int64, C.c_void_p]
         L.orc_sizeof_record.restype = C.c_size_t
+        L.orc_sizeof_cell.restype = C.c_size_t
+        vp = C.c_void_p
+        for name, args in [("orc_cell_defaults", [vp, vp]),
+                           ("orc_update_temp_dependent_coefs", [vp, vp, vp]),
+                           ("orc_activity_coefficients", [vp, vp, vp]),
+                           ("orc_total", [vp, vp]),
+                           ("orc_accumulation", [vp, vp, vp]),
+                           ("orc_accumulation_derivative", [vp, vp, C.c_double, vp]),
+                           ("orc_reaction", [vp, vp, C.c_double, vp, vp, C.c_int]),
+                           ("orc_reaction_derivative_numerical", [vp, vp, C.c_double, vp, vp]),
+                           ("orc_rreact_system", [vp, vp, vp, vp, C.c_double, vp, vp, C.c_int]),
+                           ("orc_update_kinetic_state", [vp, vp, C.c_double]),
+                           ("orc_gather", [vp, vp, C.c_int64, vp]),
+                           ("orc_scatter", [vp, vp, vp, C.c_int64])]:
+            f = getattr(L, name)
+            f.restype = None
+            f.argtypes = args
+        L.orc_rreact.restype = C.c_int
+        L.orc_rreact.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp, vp]
+        L.orc_rstep.restype = C.c_int
+        L.orc_rstep.argtypes = [vp, vp, vp, vp, C.c_double, vp, vp, vp, vp]
         L.orc_lu_decomposition.restype = C.c_int
         L.orc_lu_decomposition.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_lu_back_substitution.restype = None
@@ -111,3 +165,96 @@ def lu_solve(a, b):
         return None
     lib().orc_lu_back_substitution(a.ctypes.data, n, indx.ctypes.data, b.ctypes.data)
     return b
+
+
+class CellCtx:
+    """one oracle cell bound to a network and options (keeps the ABI structs
+    and their arrays alive)"""
+
+    def __init__(self, net, opt):
+        self.net, self.opt = net, opt
+        self.n = net.to_abi()
+        self.o = opt.to_abi()
+        self.c = OrcCell()
+        assert C.sizeof(OrcCell) == lib().orc_sizeof_cell()
+        lib().orc_cell_defaults(C.byref(self.n), C.byref(self.c))
+        self.ncomp = net.naqcomp + net.nimmobile
+
+    def gather(self, state, cell):
+        s = state.to_abi()
+        lib().orc_gather(C.byref(self.n), C.byref(s), cell, C.byref(self.c))
+        return self
+
+    def set(self, name, values):
+        a = np.ctypeslib.as_array(getattr(self.c, name))
+        v = np.asarray(values, dtype=np.float64)
+        a[:v.size] = v
+
+    def get(self, name, n):
+        return np.ctypeslib.as_array(getattr(self.c, name))[:n].copy()
+
+    def activity(self):
+        lib().orc_activity_coefficients(C.byref(self.n), C.byref(self.o), C.byref(self.c))
+
+    def total(self):
+        lib().orc_total(C.byref(self.n), C.byref(self.c))
+
+    def dtotal(self):
+        naq = self.net.naqcomp
+        return self.get("dtotal", naq * naq).reshape(naq, naq)
+
+    def reaction(self, dt, derivative=True):
+        n = self.ncomp
+        res = np.zeros(n)
+        jac = np.zeros((n, n))
+        lib().orc_reaction(C.byref(self.n), C.byref(self.c), dt, res.ctypes.data,
+                           jac.ctypes.data, 1 if derivative else 0)
+        return res, jac
+
+    def reaction_numerical(self, dt):
+        n = self.ncomp
+        res = np.zeros(n)
+        jac = np.zeros((n, n))
+        lib().orc_reaction_derivative_numerical(C.byref(self.n), C.byref(self.c), dt,
+                                                res.ctypes.data, jac.ctypes.data)
+        return res, jac
+
+    def rreact_system(self, fixed, dt, derivative=True):
+        n = self.ncomp
+        res = np.zeros(n)
+        jac = np.zeros((n, n))
+        f = np.ascontiguousarray(fixed, dtype=np.float64)
+        lib().orc_rreact_system(C.byref(self.n), C.byref(self.o), C.byref(self.c),
+                                f.ctypes.data, dt, res.ctypes.data, jac.ctypes.data,
+                                1 if derivative else 0)
+        return res, jac
+
+    def rstep(self, dt, guess=None):
+        n = self.ncomp
+        g = np.zeros(MAXCOMP)
+        naq = self.net.naqcomp
+        g[:naq] = self.get("pri_molal", naq) if guess is None else guess[:naq]
+        g[naq:n] = self.get("immobile", n - naq)
+        ns, ni, nk = C.c_int(0), C.c_int(0), C.c_int(0)
+        rec = Records(1)
+        err = lib().orc_rstep(C.byref(self.n), C.byref(self.o), C.byref(self.c),
+                              g.ctypes.data, dt, C.byref(ns), C.byref(ni), C.byref(nk),
+                              rec.ptr)
+        return err, ns.value, ni.value
+
+    def rreact(self, dt, guess=None):
+        """one RReact solve from the cell's totals (no RStep sub-stepping and
+        no small-total clamp); returns (ierror, iterations, fail_reason)"""
+        n = self.ncomp
+        naq = self.net.naqcomp
+        g = np.zeros(MAXCOMP)
+        g[:naq] = self.get("pri_molal", naq) if guess is None else guess[:naq]
+        g[naq:n] = self.get("immobile", n - naq)
+        ni, why = C.c_int(0), C.c_int(0)
+        rec = Records(1)
+        err = lib().orc_rreact(C.byref(self.n), C.byref(self.o), C.byref(self.c),
+                               g.ctypes.data, dt, C.byref(ni), rec.ptr, C.byref(why))
+        return err, ni.value, why.value
+
+    def update_kinetic_state(self, dt):
+        lib().orc_update_kinetic_state(C.byref(self.n), C.byref(self.c), dt)

@@ -910,6 +910,7 @@ double orc_record_rel_residual(const pfchem_network* net,
 }
 
 size_t orc_sizeof_record(void) { return sizeof(orc_substep_record); }
+size_t orc_sizeof_cell(void) { return sizeof(orc_cell); }
 
 /* totals / secondary molalities / dtotal of one cell at its stored free
    molality (RTotal), for tests */

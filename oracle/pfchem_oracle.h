@@ -125,6 +125,7 @@ double orc_record_rel_residual(const pfchem_network* net,
                                const pfchem_state* s, int64_t cell,
                                const orc_substep_record* rec);
 size_t orc_sizeof_record(void);
+size_t orc_sizeof_cell(void);
 void orc_cell_total(const pfchem_network* net, const pfchem_options* opt,
                     const pfchem_state* s, int64_t cell, double* total,
                     double* sec_molal, double* dtotal);
