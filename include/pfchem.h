@@ -60,9 +60,13 @@ enum {
                                           (reaction.F90:3869-3895, 3662)   */
   PFCHEM_CELL_MAX_CUTS_SINGULAR = 2,   /* cuts exhausted, last RReact had a
                                           zero Jacobian row (utility.F90:629)*/
-  PFCHEM_CELL_MAX_CUTS_ACTIVITY = 3    /* cuts exhausted, last RReact had an
+  PFCHEM_CELL_MAX_CUTS_ACTIVITY = 3,   /* cuts exhausted, last RReact had an
                                           activity-coefficient failure
                                           (reaction.F90:4421-4431,4477-4486) */
+  PFCHEM_CELL_SENS_UNSUPPORTED = 4,    /* pfchem_tangent/adjoint: the cell is
+                                          dry (no sensitivity computed)      */
+  PFCHEM_CELL_TOO_MANY_SUBSTEPS = 5    /* pfchem_tangent/adjoint: more than 8
+                                          accepted sub-steps to record       */
 };
 
 /* activity coefficient update frequency / algorithm
@@ -316,6 +320,38 @@ typedef struct pfchem_launch {
 } pfchem_launch;
 int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg);
 int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg);
+
+/* ------------------------------------------------------------------ */
+/* sensitivities of one step                                           */
+/* ------------------------------------------------------------------ */
+/* Derivatives of the per-cell objective
+     f = sum_n w_vf[n] * mnrl_volfrac_n(end) + sum_k w_sec[k] * sec_molal_k(end)
+   (end of one RStep of length dt from the given state, which is read only)
+   with respect to p = (log10 kinmnrl_rate_constant[0..nkinmnrl),
+   log10 K of the complexes eqcplx_logK[0..neqcplx)), through the converged
+   Newton system of every accepted sub-step (implicit-function theorem).
+   The reference has no counterpart; the derivatives are those of RStep /
+   RReact (reaction.F90:3564-4055) and MineralUpdateKineticState
+   (reaction_mineral.F90:1456-1525).  jacobian: 1 = exact dF/dm,
+   0 = as the reference assembles it (mineral block times rho/1000,
+   reaction_mineral.F90:945).  Supported: no immobile species, plain TST
+   kinetics (optional activation energy), activity coefficients fixed
+   during the step (not PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER), up to 32
+   primaries and 16 minerals; otherwise PFCHEM_ERR_UNSUPPORTED.  state,
+   f_cell, status: device pointers; weights and dp: host arrays. */
+
+/* tangent along dp[nkinmnrl + neqcplx]: df_cell[cell] = df/dp . dp.
+   f_cell (optional) receives f.  Asynchronous on `stream`. */
+int pfchem_tangent(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
+                   const double* w_vf, const double* w_sec, const double* dp, int jacobian,
+                   double* df_cell, double* f_cell, int32_t* status, void* stream);
+
+/* adjoint: grad[nkinmnrl + neqcplx] (host) = sum over cells with status 0 of
+   df/dp; grad_cell (device, optional, [q * ncell + cell]) the per-cell
+   gradients.  Synchronizes `stream` before returning. */
+int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
+                   const double* w_vf, const double* w_sec, int jacobian, double* grad,
+                   double* grad_cell, double* f_cell, int32_t* status, void* stream);
 
 #ifdef __cplusplus
 }

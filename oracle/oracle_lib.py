@@ -258,3 +258,49 @@ class CellCtx:
 
     def update_kinetic_state(self, dt):
         lib().orc_update_kinetic_state(C.byref(self.n), C.byref(self.c), dt)
+
+
+def _sens_lib():
+    L = lib()
+    if not getattr(L, "_sens_bound", False):
+        vp = C.c_void_p
+        L.orc_adjoint_batch.restype = C.c_int64
+        L.orc_adjoint_batch.argtypes = [vp, vp, vp, C.c_int64, C.c_double, vp, vp, C.c_int,
+                                        C.c_int, vp, vp, vp]
+        L.orc_tangent_batch.restype = C.c_int64
+        L.orc_tangent_batch.argtypes = [vp, vp, vp, C.c_int64, C.c_double, vp, vp, vp, C.c_int,
+                                        C.c_int, vp, vp, vp]
+        L._sens_bound = True
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def adjoint_batch(net, opt, state, dt, wv, ws, exact=True, nthreads=0):
+    """(sum over cells of d f / d p, f per cell, status per cell) with
+    p = (log10 k of the kinetic minerals, log10 K of the complexes)"""
+    n, o, s = net.to_abi(), opt.to_abi(), state.to_abi()
+    wv, ws = _f64(wv), _f64(ws)
+    grad = np.zeros(max(net.nkinmnrl + net.neqcplx, 1))
+    f = np.zeros(state.ncell)
+    st = np.zeros(state.ncell, dtype=np.int32)
+    _sens_lib().orc_adjoint_batch(C.byref(n), C.byref(o), C.byref(s), state.ncell, dt,
+                                  wv.ctypes.data, ws.ctypes.data, 1 if exact else 0, nthreads,
+                                  grad.ctypes.data, f.ctypes.data, st.ctypes.data)
+    return grad[:net.nkinmnrl + net.neqcplx], f, st
+
+
+def tangent_batch(net, opt, state, dt, wv, ws, dp, exact=True, nthreads=0):
+    """(d f per cell along dp, f per cell, status per cell)"""
+    n, o, s = net.to_abi(), opt.to_abi(), state.to_abi()
+    wv, ws, dp = _f64(wv), _f64(ws), _f64(dp)
+    df = np.zeros(state.ncell)
+    f = np.zeros(state.ncell)
+    st = np.zeros(state.ncell, dtype=np.int32)
+    _sens_lib().orc_tangent_batch(C.byref(n), C.byref(o), C.byref(s), state.ncell, dt,
+                                  wv.ctypes.data, ws.ctypes.data, dp.ctypes.data,
+                                  1 if exact else 0, nthreads, df.ctypes.data, f.ctypes.data,
+                                  st.ctypes.data)
+    return df, f, st

@@ -568,6 +568,9 @@ int orc_rsolve(const double* res, double* jac, const double* conc,
    ||r||/||r0||, max relative change (NaN before the update) */
 void (*orc_trace)(int it, double two_norm, double rel, double max_rel_change) = 0;
 
+/* accepted sub-steps of the current RStep (per thread; see orc_substep_log) */
+__thread orc_substep_log* orc_log = 0;
+
 /* body of one RReact iterate: reaction.F90:3899-3916 */
 void orc_rreact_system(const pfchem_network* net, const pfchem_options* opt,
                        orc_cell* c, const double* fixed_accum, double dt,
@@ -736,10 +739,14 @@ int orc_rstep(const pfchem_network* net, const pfchem_options* opt,
   double target_time = dt, cumulative_time = 0.0, tran_dt = dt;
   int num_const = 0, num_cuts = 0, last_reason = 0, attempts = 0;
   orc_substep_record tmp;
+  double t_start[ORC_MAXCOMP];
+  if (orc_log) orc_log->n = orc_log->overflow = 0;
   for (;;) {
     if (cumulative_time >= target_time) break;
     if (++attempts > ORC_MAX_RREACT_ATTEMPTS) return last_reason ? last_reason : 1;
     int inner = 0, reason = 0;
+    if (orc_log)
+      for (int i = 0; i < naq; ++i) t_start[i] = c->total[i];
     int ierr = orc_rreact(net, opt, c, guess, tran_dt, &inner, rec ? &tmp : NULL,
                           &reason);
     *num_iterations += inner;
@@ -751,6 +758,19 @@ int orc_rstep(const pfchem_network* net, const pfchem_options* opt,
       num_const = 0;
     } else {
       if (rec) *rec = tmp;
+      if (orc_log) {
+        if (orc_log->n < ORC_LOG_MAX_SUBSTEPS) {
+          const int k = orc_log->n++;
+          orc_log->dt[k] = tran_dt;
+          for (int i = 0; i < naq; ++i) {
+            orc_log->t0[k][i] = t_start[i];
+            orc_log->m[k][i] = c->pri_molal[i];
+          }
+          for (int m = 0; m < net->nkinmnrl; ++m) orc_log->vf0[k][m] = c->mnrl_volfrac[m];
+        } else {
+          orc_log->overflow = 1;
+        }
+      }
       orc_update_kinetic_state(net, c, tran_dt);
       cumulative_time += tran_dt;
       ++*num_sub_steps;

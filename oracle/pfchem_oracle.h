@@ -66,6 +66,19 @@ typedef struct orc_substep_record {
   double final_rel_residual;          /* ||F(x*)||/||F(x0)|| at exit       */
 } orc_substep_record;
 
+/* accepted sub-steps of one RStep: start totals, mineral volume fractions
+   before the kinetic update, solution and sub-step size (for the adjoint) */
+#define ORC_LOG_MAX_SUBSTEPS 64
+typedef struct orc_substep_log {
+  int n, overflow;
+  double dt[ORC_LOG_MAX_SUBSTEPS];
+  double t0[ORC_LOG_MAX_SUBSTEPS][ORC_MAXCOMP];
+  double vf0[ORC_LOG_MAX_SUBSTEPS][ORC_MAXMNRL];
+  double m[ORC_LOG_MAX_SUBSTEPS][ORC_MAXCOMP];
+} orc_substep_log;
+/* when non-NULL, orc_rstep fills it (thread-local) */
+extern __thread orc_substep_log* orc_log;
+
 /* ---- building blocks (exported for unit tests) ---- */
 void orc_cell_defaults(const pfchem_network* net, orc_cell* c);
 void orc_update_temp_dependent_coefs(const pfchem_network* net,
@@ -125,6 +138,33 @@ double orc_record_rel_residual(const pfchem_network* net,
                                const pfchem_state* s, int64_t cell,
                                const orc_substep_record* rec);
 size_t orc_sizeof_record(void);
+
+/* ---- sensitivities of one step (pfchem_oracle_adjoint.c) ----
+   Objective of a cell: f = sum_m wv[m] VF_m(end) + sum_k ws[k] m_k(end)
+   (end-of-step mineral volume fractions and secondary molalities).
+   Parameters p = (log10 k_m for the nkinmnrl minerals, log10 K_k for the
+   neqcplx complexes).  exact_jacobian = 0 uses the reference's analytic
+   Jacobian as assembled (its mineral block carries rho/1000,
+   reaction_mineral.F90:945); 1 the exact derivative.  Return 0, or
+   1 step failed, 2 unsupported (immobile species, activity coefficients
+   updated inside Newton, mineral affinity options), 3 too many sub-steps. */
+int orc_cell_adjoint(const pfchem_network* net, const pfchem_options* opt,
+                     orc_cell* c, double dt, const double* wv, const double* ws,
+                     int exact_jacobian, double* f, double* grad);
+int orc_cell_tangent(const pfchem_network* net, const pfchem_options* opt,
+                     orc_cell* c, double dt, const double* wv, const double* ws,
+                     const double* dp, int exact_jacobian, double* f, double* df);
+/* over cells: grad_sum[nkinmnrl + neqcplx] += per-cell gradients; f[cell]
+   and status[cell] per cell (may be NULL).  Returns failed cells. */
+int64_t orc_adjoint_batch(const pfchem_network* net, const pfchem_options* opt,
+                          const pfchem_state* s, int64_t ncell, double dt,
+                          const double* wv, const double* ws, int exact_jacobian,
+                          int nthreads, double* grad_sum, double* f, int32_t* status);
+int64_t orc_tangent_batch(const pfchem_network* net, const pfchem_options* opt,
+                          const pfchem_state* s, int64_t ncell, double dt,
+                          const double* wv, const double* ws, const double* dp,
+                          int exact_jacobian, int nthreads, double* df, double* f,
+                          int32_t* status);
 size_t orc_sizeof_cell(void);
 void orc_cell_total(const pfchem_network* net, const pfchem_options* opt,
                     const pfchem_state* s, int64_t cell, double* total,

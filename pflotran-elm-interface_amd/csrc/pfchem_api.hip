@@ -32,6 +32,15 @@ struct pfchem_ctx {
   // host staging (pfchem_step_host)
   void* stage = nullptr;
   size_t stage_bytes = 0;
+  // sensitivities: weights / direction (device + pinned mirror), per-cell
+  // gradient and status scratch, gradient sum
+  double* sens_in = nullptr;     // device: w_vf[nm], w_sec[nc], dp[nm+nc]
+  double* sens_in_h = nullptr;   // pinned host mirror
+  double* sens_sum = nullptr;    // device: sum of per-cell gradients
+  double* sens_sum_h = nullptr;  // pinned host mirror
+  hipEvent_t sens_copied = nullptr;
+  void* sens_work = nullptr;     // per-cell gradients + status when not given
+  size_t sens_work_bytes = 0;
   std::string err;
 };
 
@@ -303,6 +312,12 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->dnet_dev) (void)hipFree(ctx->dnet_dev);
   if (ctx->dstate_ring) (void)hipFree(ctx->dstate_ring);
   if (ctx->hstate_ring) (void)hipHostFree(ctx->hstate_ring);
+  if (ctx->sens_in) (void)hipFree(ctx->sens_in);
+  if (ctx->sens_in_h) (void)hipHostFree(ctx->sens_in_h);
+  if (ctx->sens_sum) (void)hipFree(ctx->sens_sum);
+  if (ctx->sens_sum_h) (void)hipHostFree(ctx->sens_sum_h);
+  if (ctx->sens_copied) (void)hipEventDestroy(ctx->sens_copied);
+  if (ctx->sens_work) (void)hipFree(ctx->sens_work);
   delete ctx;
   return PFCHEM_OK;
 }
@@ -492,6 +507,139 @@ int pfchem_step_host(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* h, doub
   if (h->status)
     for (int64_t c = 0; c < ncell; ++c)
       if (h->status[c]) return fail(ctx, PFCHEM_ERR_CELLS, "cells failed");
+  return PFCHEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// weights / direction to the device through the pinned mirror; the mirror
+// is rewritten only after the previous call's copy has completed
+int sens_stage(pfchem_ctx* ctx, const double* wv, const double* ws, const double* dp,
+               hipStream_t s) {
+  const int nm = ctx->dnet.nm, nc = ctx->dnet.nc, np = nm + nc;
+  const size_t nin = (size_t)2 * np + 1;
+  hipError_t e = hipSuccess;
+  if (!ctx->sens_in) {
+    if (hipMalloc(&ctx->sens_in, nin * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&ctx->sens_in_h, nin * sizeof(double)) != hipSuccess ||
+        hipMalloc(&ctx->sens_sum, (np + 1) * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&ctx->sens_sum_h, (np + 1) * sizeof(double)) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->sens_copied, hipEventDisableTiming) != hipSuccess)
+      return fail(ctx, PFCHEM_ERR_NOMEM, "sensitivity buffers");
+  } else {
+    e = hipEventSynchronize(ctx->sens_copied);
+    if (e != hipSuccess) return hip_fail(ctx, e, "sensitivity staging");
+  }
+  double* h = ctx->sens_in_h;
+  for (int q = 0; q < nm; ++q) h[q] = wv ? wv[q] : 0.0;
+  for (int k = 0; k < nc; ++k) h[nm + k] = ws ? ws[k] : 0.0;
+  for (int q = 0; q < np; ++q) h[np + q] = dp ? dp[q] : 0.0;
+  e = hipMemcpyAsync(ctx->sens_in, h, nin * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(ctx->sens_copied, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "sensitivity staging");
+  return PFCHEM_OK;
+}
+
+int sens_check(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
+               DevState* d) {
+  if (!ctx) return PFCHEM_ERR_INVALID;
+  if (ncell < 0) return fail(ctx, PFCHEM_ERR_INVALID, "ncell < 0");
+  if (!(dt > 0.0)) return fail(ctx, PFCHEM_ERR_INVALID, "dt must be > 0");
+  if (!sens_supported(ctx->dnet, ctx->opt))
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED,
+                "sensitivities need an aqueous network with plain TST minerals and "
+                "activity coefficients fixed during the step");
+  int rc = to_dev_state(ctx, state, ncell, d);
+  if (rc) return rc;
+  if (ncell == 0) return PFCHEM_OK;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  return PFCHEM_OK;
+}
+
+DevState* sens_state_slot(pfchem_ctx* ctx, const DevState& d, hipStream_t s, hipError_t* e) {
+  DevState* slot = ctx->dstate_ring + ctx->ring_next;
+  DevState* hslot = ctx->hstate_ring + ctx->ring_next;
+  ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
+  *hslot = d;
+  *e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+  return slot;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfchem_tangent(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
+                   const double* w_vf, const double* w_sec, const double* dp, int jacobian,
+                   double* df_cell, double* f_cell, int32_t* status, void* stream) {
+  DevState d;
+  int rc = sens_check(ctx, ncell, state, dt, &d);
+  if (rc || ncell == 0) return rc;
+  if (!df_cell || !dp) return fail(ctx, PFCHEM_ERR_INVALID, "tangent needs dp and df_cell");
+  hipStream_t s = (hipStream_t)stream;
+  rc = sens_stage(ctx, w_vf, w_sec, dp, s);
+  if (rc) return rc;
+  hipError_t e;
+  DevState* slot = sens_state_slot(ctx, d, s, &e);
+  if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+  const int np = ctx->dnet.nm + ctx->dnet.nc;
+  e = launch_sens(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, ctx->sens_in,
+                  ctx->sens_in + ctx->dnet.nm, ctx->sens_in + np, 0, jacobian ? 1 : 0, f_cell,
+                  df_cell, ncell, status, nullptr, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "tangent launch");
+  return PFCHEM_OK;
+}
+
+int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
+                   const double* w_vf, const double* w_sec, int jacobian, double* grad,
+                   double* grad_cell, double* f_cell, int32_t* status, void* stream) {
+  DevState d;
+  int rc = sens_check(ctx, ncell, state, dt, &d);
+  const int np = ctx ? ctx->dnet.nm + ctx->dnet.nc : 0;
+  if (rc) return rc;
+  if (ncell == 0) {
+    if (grad)
+      for (int q = 0; q < np; ++q) grad[q] = 0.0;
+    return PFCHEM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  rc = sens_stage(ctx, w_vf, w_sec, nullptr, s);
+  if (rc) return rc;
+  // per-cell gradients and status: caller's or the context's scratch
+  size_t need = 0;
+  if (!grad_cell) need += (size_t)np * ncell * sizeof(double);
+  if (!status) need += (size_t)ncell * sizeof(int32_t);
+  if (need > ctx->sens_work_bytes) {
+    if (ctx->sens_work) (void)hipFree(ctx->sens_work);
+    ctx->sens_work = nullptr;
+    ctx->sens_work_bytes = 0;
+    if (hipMalloc(&ctx->sens_work, need) != hipSuccess)
+      return fail(ctx, PFCHEM_ERR_NOMEM, "sensitivity scratch");
+    ctx->sens_work_bytes = need;
+  }
+  char* wp = (char*)ctx->sens_work;
+  double* gc = grad_cell;
+  if (!gc) {
+    gc = (double*)wp;
+    wp += (size_t)np * ncell * sizeof(double);
+  }
+  int32_t* st = status ? status : (int32_t*)wp;
+  hipError_t e;
+  DevState* slot = sens_state_slot(ctx, d, s, &e);
+  if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+  e = launch_sens(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, ctx->sens_in,
+                  ctx->sens_in + ctx->dnet.nm, nullptr, 1, jacobian ? 1 : 0, f_cell, gc, ncell,
+                  st, ctx->sens_sum, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->sens_sum_h, ctx->sens_sum, np * sizeof(double),
+                       hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "adjoint");
+  if (grad)
+    for (int q = 0; q < np; ++q) grad[q] = ctx->sens_sum_h[q];
   return PFCHEM_OK;
 }
 

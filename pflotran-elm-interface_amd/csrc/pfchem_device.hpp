@@ -97,10 +97,25 @@ __device__ __forceinline__ double sat_of(const DevState& s, int64_t c) {
   return s.saturation[c];
 }
 
+// Loads and stores through the global address space.  Network and state
+// pointers reach the kernels inside descriptors, so the compiler sees
+// generic pointers and would emit FLAT accesses; those count against the
+// same wait counter as LDS operations, so every such read inside a loop of
+// LDS atomics would wait for all atomics issued before it.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)(p) = v;
+}
+
 // ReactionInterpolateLogK (reaction_aux.F90:1285-1312) for one entry
 __device__ __forceinline__ double interp_logk(const double* b, double temp) {
   double tk = temp + 273.15;
-  return b[0] * log(tk) + b[1] + b[2] * tk + b[3] / tk + b[4] / (tk * tk);
+  return gld(b) * log(tk) + gld(b + 1) + gld(b + 2) * tk + gld(b + 3) / tk +
+         gld(b + 4) / (tk * tk);
 }
 
 // Debye-Hueckel ln(gamma) * 1/ln10 -> gamma (reaction.F90:4586-4592)

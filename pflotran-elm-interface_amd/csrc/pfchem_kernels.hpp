@@ -35,4 +35,12 @@ hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* d
                             unsigned long long* cursors, int num_cus, int nxcd,
                             hipStream_t stream, bool has_status);
 
+// sensitivities (tangent: mode 0, out = df per cell; adjoint: mode 1, out =
+// per-cell gradients [q * ldo + cell], sum = their sum over converged cells)
+int sens_supported(const DevNet& n, const pfchem_options& opt);
+hipError_t launch_sens(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
+                       double dt, const double* wv, const double* ws, const double* dp, int mode,
+                       int jacobian, double* fcell, double* out, int64_t ldo, int32_t* status,
+                       double* sum, hipStream_t stream);
+
 }  // namespace pfchem

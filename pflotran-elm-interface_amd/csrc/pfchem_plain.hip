@@ -7,6 +7,7 @@
 // the FP64 reference point the batched kernels are measured against.
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
+#include "pfchem_lu.hpp"
 
 namespace pfchem {
 namespace {
@@ -148,70 +149,6 @@ __device__ void plain_minerals(const DevNet& n, const DevState& s, int64_t c,
   }
 }
 
-// LUDecomposition1 (utility.F90:597-688) + LUBackSubstitution (:692-735)
-template <int MAXN>
-__device__ int plain_lu_solve(double* a, int N, double* b) {
-  double vv[MAXN];
-  int indx[MAXN];
-  int imax = 0;
-  for (int i = 0; i < N; ++i) {
-    double amax = 0.0;
-    for (int j = 0; j < N; ++j) amax = fmax(amax, fabs(a[i * N + j]));
-    if (amax <= 0.0) return 1;
-    vv[i] = 1.0 / amax;
-  }
-  for (int j = 0; j < N; ++j) {
-    for (int i = 0; i < j; ++i) {
-      double sum = a[i * N + j];
-      for (int k = 0; k < i; ++k) sum -= a[i * N + k] * a[k * N + j];
-      a[i * N + j] = sum;
-    }
-    double amax = 0.0;
-    for (int i = j; i < N; ++i) {
-      double sum = a[i * N + j];
-      for (int k = 0; k < j; ++k) sum -= a[i * N + k] * a[k * N + j];
-      a[i * N + j] = sum;
-      const double dum = vv[i] * fabs(sum);
-      if (dum >= amax) {
-        imax = i;
-        amax = dum;
-      }
-    }
-    if (j != imax) {
-      for (int k = 0; k < N; ++k) {
-        const double t = a[imax * N + k];
-        a[imax * N + k] = a[j * N + k];
-        a[j * N + k] = t;
-      }
-      vv[imax] = vv[j];
-    }
-    indx[j] = imax;
-    if (a[j * N + j] == 0.0) a[j * N + j] = 1.0e-20;
-    if (j != N - 1) {
-      const double dum = 1.0 / a[j * N + j];
-      for (int i = j + 1; i < N; ++i) a[i * N + j] *= dum;
-    }
-  }
-  int ii = -1;
-  for (int i = 0; i < N; ++i) {
-    const int ll = indx[i];
-    double sum = b[ll];
-    b[ll] = b[i];
-    if (ii >= 0) {
-      for (int j = ii; j < i; ++j) sum -= a[i * N + j] * b[j];
-    } else if (sum != 0.0) {
-      ii = i;
-    }
-    b[i] = sum;
-  }
-  for (int i = N - 1; i >= 0; --i) {
-    double sum = b[i];
-    for (int j = i + 1; j < N; ++j) sum -= a[i * N + j] * b[j];
-    b[i] = sum / a[i * N + i];
-  }
-  return 0;
-}
-
 // RActivityCoefficients LAG (reaction.F90:4555-4611) with the cell's
 // current pri/sec molalities; sec_act_coef written to the SoA state.
 template <int MAXN>
@@ -332,7 +269,7 @@ __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
     if (n.log_form)
       for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j) w.J[i * N + j] *= prev[j];
-    if (plain_lu_solve<MAXN>(w.J, N, upd)) return kSingular;
+    if (crout_lu_solve<MAXN>(w.J, N, upd)) return kSingular;
     double mrc = -1.0;
     if (n.log_form) {
       for (int i = 0; i < N; ++i) {

@@ -206,8 +206,9 @@ struct MineralRate {
 // exponent 1/sigma of QK in the affinity 1 - QK^(1/sigma)
 __device__ __forceinline__ double affinity_sigma(const DevNet& n, int m) {
   if (n.has_temkin)
-    return n.has_minscale ? n.mnrl_minscale[m] * n.mnrl_temkin[m] : n.mnrl_temkin[m];
-  return n.mnrl_minscale[m];
+    return n.has_minscale ? gld(n.mnrl_minscale + m) * gld(n.mnrl_temkin + m)
+                          : gld(n.mnrl_temkin + m);
+  return gld(n.mnrl_minscale + m);
 }
 
 __device__ __noinline__ double pow_noinline(double a, double b) { return pow(a, b); }
@@ -217,12 +218,12 @@ __device__ __noinline__ double pow_noinline(double a, double b) { return pow(a, 
 __device__ __forceinline__ void rate_options(const DevNet& n, int m, double QK, double aff,
                                              double sgn, double kr, double Im_const,
                                              MineralRate* r) {
-  const double p = n.has_affpower ? n.mnrl_affpower[m] : 1.0;
+  const double p = n.has_affpower ? gld(n.mnrl_affpower + m) : 1.0;
   r->Im = n.has_affpower ? Im_const * sgn * pow_noinline(fabs(aff), p) * kr
                          : Im_const * sgn * fabs(aff) * kr;
   double d = n.has_affpower ? -r->Im * p / fabs(aff) : -Im_const * kr;
   if (n.has_temkin || n.has_minscale) d = d * (1.0 / affinity_sigma(n, m)) / QK * (1.0 - aff);
-  const double lim = n.has_limiter ? n.mnrl_limiter[m] : 0.0;
+  const double lim = n.has_limiter ? gld(n.mnrl_limiter + m) : 0.0;
   if (lim > 0.0) {
     const double d2 = 1.0 + (1.0 - aff) / lim;
     d *= (1.0 + QK / lim / d2) / d2;
@@ -242,17 +243,22 @@ __device__ __forceinline__ MineralRate mineral_rate(const DevNet& n, int m, doub
   double aff = plain_aff ? 1.0 - QK : 1.0 - pow_noinline(QK, 1.0 / affinity_sigma(n, m));
   const double sgn = copysign(1.0, aff);
   if (!(volfrac > 0.0 || sgn < 0.0)) return r;
-  if (n.has_irrev && n.mnrl_irrev[m] == 1 && sgn < 0.0) return r;
-  if (n.has_threshold && n.mnrl_threshold[m] > 0.0 && sgn < 0.0 && QK < n.mnrl_threshold[m])
-    return r;
-  const double lim = n.has_limiter ? n.mnrl_limiter[m] : 0.0;
+  if (n.has_irrev && gld(n.mnrl_irrev + m) == 1 && sgn < 0.0) return r;
+  if (n.has_threshold) {
+    const double th = gld(n.mnrl_threshold + m);
+    if (th > 0.0 && sgn < 0.0 && QK < th) return r;
+  }
+  const double lim = n.has_limiter ? gld(n.mnrl_limiter + m) : 0.0;
   if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
   double arr = 1.0;
-  if (n.has_Ea && n.mnrl_Ea[m] > 0.0)
-    arr = exp(n.mnrl_Ea[m] / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
-  const double kr = n.mnrl_k[m] * arr;
+  if (n.has_Ea) {
+    const double Ea = gld(n.mnrl_Ea + m);
+    if (Ea > 0.0)
+      arr = exp(Ea / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
+  }
+  const double kr = gld(n.mnrl_k + m) * arr;
   double Im_const = -area;
-  if (n.has_minscale) Im_const = Im_const / n.mnrl_minscale[m];
+  if (n.has_minscale) Im_const = Im_const / gld(n.mnrl_minscale + m);
   r.active = true;
   if (plain_aff && !n.has_affpower && !(lim > 0.0)) {
     r.Im = Im_const * sgn * fabs(aff) * kr;
@@ -265,22 +271,85 @@ __device__ __forceinline__ MineralRate mineral_rate(const DevNet& n, int m, doub
 
 // mass-action ln QK of complex k from the team's ln-activity vector
 __device__ __forceinline__ double complex_lnqk(const DevNet& n, int k, const double* lnact,
-                                               double ln_aw, double temp, int* ncs_out,
-                                               const int** ids_out, const double** st_out) {
+                                               double ln_aw, double temp) {
   const int ms = n.cplx_ms;
-  double logK = n.cplx_logK[k];
-  if (!n.isothermal && n.has_cplx_logKcoef) logK = interp_logk(n.cplx_logKcoef + 5 * k, temp);
+  double logK = n.isothermal || !n.has_cplx_logKcoef ? gld(n.cplx_logK + k)
+                                                     : interp_logk(n.cplx_logKcoef + 5 * k, temp);
   double lnQK = -logK * kLogToLn;
-  if (n.has_cplx_h2o) lnQK += n.cplx_h2o[k] * ln_aw;
+  if (n.has_cplx_h2o) lnQK += gld(n.cplx_h2o + k) * ln_aw;
   const int* ids = n.cplx_ids + k * (ms + 1);
   const double* st = n.cplx_st + k * ms;
-  const int ncs = ids[0];
-  for (int q = 0; q < ncs; ++q) lnQK += st[q] * lnact[ids[1 + q]];
-  *ncs_out = ncs;
-  *ids_out = ids;
-  *st_out = st;
+  const int ncs = gld(ids);
+  for (int q = 0; q < ncs; ++q) lnQK += gld(st + q) * lnact[gld(ids + 1 + q)];
   return lnQK;
 }
+
+// Species slots per reaction row held in registers; rows with more species
+// take a slower path for the rest.
+constexpr int kRowSlots = 4;
+
+// One reaction row (species ids and stoichiometry, the compressed
+// eqcplxspecid / eqcplxstoich layout of reaction_database.F90:1704-1772)
+// staged in registers together with the row scales of its species.
+template <int S>
+struct RxnRow {
+  int ncs;
+  int id[S];
+  double nu[S], rs[S];
+
+  // stage the row; returns init + sum nu_a ln a_a (the mass-action sum)
+  __device__ __forceinline__ double load(const int* ids, const double* st, const double* lnact,
+                                         const double* rsv, double init) {
+    ncs = gld(ids);
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      id[a] = a < ncs ? gld(ids + 1 + a) : 0;
+      nu[a] = a < ncs ? gld(st + a) : 0.0;
+    }
+    double sum = init;
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      rs[a] = 0.0;
+      if (a < ncs) {
+        sum += nu[a] * lnact[id[a]];
+        rs[a] = rsv[id[a]];
+      }
+    }
+    for (int a = S; a < ncs; ++a) sum += gld(st + a) * lnact[gld(ids + 1 + a)];
+    return sum;
+  }
+
+  // acc_i += nu_i v; J_ij += nu_i nu_j w * (row scale of i)
+  template <int NM, typename JT>
+  __device__ __forceinline__ void scatter(const int* ids, const double* st, double v, double w,
+                                          bool jac, double* acc, JT* Jt,
+                                          const double* rsv) const {
+#pragma unroll
+    for (int a = 0; a < S; ++a) {
+      if (a < ncs) {
+        atomicAdd(&acc[id[a]], nu[a] * v);
+        if (jac) {
+          const double wi = nu[a] * w * rs[a];
+#pragma unroll
+          for (int b = 0; b < S; ++b)
+            if (b < ncs) atomicAdd(&Jt[id[a] * (NM + 1) + id[b]], (JT)(wi * nu[b]));
+        }
+      }
+    }
+    if (ncs > S) {
+      for (int a = 0; a < ncs; ++a) {
+        const int i = gld(ids + 1 + a);
+        const double nua = gld(st + a);
+        if (a >= S) atomicAdd(&acc[i], nua * v);
+        if (!jac) continue;
+        const double wi = nua * w * rsv[i];
+        for (int b = 0; b < ncs; ++b)
+          if (a >= S || b >= S)
+            atomicAdd(&Jt[i * (NM + 1) + gld(ids + 1 + b)], (JT)(wi * gld(st + b)));
+      }
+    }
+  }
+};
 
 // Crout LU with implicit partial pivoting and back substitution
 // (LUDecomposition1 / LUBackSubstitution, utility.F90:597-735) on the
@@ -358,11 +427,20 @@ constexpr int kClaimAlign = 16;
 
 enum Outcome : int { kContinue = 0, kConverged = 1, kFailMaxIt = 2, kFailSingular = 3 };
 
+// waves per SIMD the register allocation is sized for
+#ifndef PFCHEM_TEAM_WAVES_PER_EU
+#define PFCHEM_TEAM_WAVES_PER_EU 2
+#endif
+
 template <int L, int R, typename JT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PFCHEM_TEAM_WAVES_PER_EU)))
 team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                  unsigned long long* __restrict__ cursors, int nxcd, int rescue_only) {
   constexpr int NM = L * R;
+  // reaction rows per lane staged in registers: complexes per chunk (more
+  // than one spills registers) and kinetic minerals
+  constexpr int KC = 1;
+  constexpr int KM = (kMaxMnrl + L - 1) / L < 4 ? (kMaxMnrl + L - 1) / L : 4;
   const DevNet& n = *np;
   const DevState& s = *sp;
   extern __shared__ __align__(16) char smem[];
@@ -455,39 +533,39 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     if (cell < 0 && !exhausted && got >= 0) {
       if (got >= ncell) {
         exhausted = true;
-      } else if (rescue_only && s.status[got] != kEscalate) {
+      } else if (rescue_only && gld(s.status + (got)) != kEscalate) {
         // FP64 rescue pass: only cells the FP32 pass handed over
       } else {
         cell = got;
         const int64_t c = cell;
-        por = s.porosity[c];
-        sat = s.saturation[c];
-        den = s.den_kg[c];
-        vol = s.volume[c];
-        temp = s.temperature ? s.temperature[c] : 25.0;
-        ln_aw = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
+        por = gld(s.porosity + (c));
+        sat = gld(s.saturation + (c));
+        den = gld(s.den_kg + (c));
+        vol = gld(s.volume + (c));
+        temp = s.temperature ? gld(s.temperature + (c)) : 25.0;
+        ln_aw = s.ln_act_h2o ? gld(s.ln_act_h2o + (c)) : 0.0;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int i = r * L + t;
           if (i < naq) {
-            x[r] = s.pri_molal[i * ld + c];
-            total[r] = s.total[i * ld + c];
-            lngam[r] = log(s.pri_act_coef[i * ld + c]);
+            x[r] = gld(s.pri_molal + (i * ld + c));
+            total[r] = gld(s.total + (i * ld + c));
+            lngam[r] = log(gld(s.pri_act_coef + (i * ld + c)));
           } else if (i < N) {
-            x[r] = s.immobile[(i - naq) * ld + c];
+            x[r] = gld(s.immobile + ((i - naq) * ld + c));
             total[r] = x[r];
           }
         }
         sec_I = sec_sum = 0.0;
         for (int k = t; k < nc; k += L) {
-          lng[k] = log(s.sec_act_coef[k * ld + c]);
-          const double m_k = s.sec_molal[k * ld + c];
-          sec_I += m_k * n.cplx_Z[k] * n.cplx_Z[k];
+          lng[k] = log(gld(s.sec_act_coef + (k * ld + c)));
+          const double m_k = gld(s.sec_molal + (k * ld + c));
+          sec_I += m_k * gld(n.cplx_Z + k) * gld(n.cplx_Z + k);
           sec_sum += m_k;
         }
         for (int m = t; m < nm; m += L) {
-          vf[m] = s.mnrl_volfrac[m * ld + c];
-          area[m] = s.mnrl_area[m * ld + c];
+          vf[m] = gld(s.mnrl_volfrac + (m * ld + c));
+          area[m] = gld(s.mnrl_area + (m * ld + c));
           rate[m] = 0.0;
         }
         nsub = nit_total = status = 0;
@@ -497,12 +575,12 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             const int i = r * L + t;
-            if (i < naq) s.pri_molal[i * ld + c] = total[r] / den * 1.0e3;
+            if (i < naq) gst(s.pri_molal + (i * ld + c), total[r] / den * 1.0e3);
           }
           if (t == 0) {
-            if (s.num_iterations) s.num_iterations[c] = 0;
-            if (s.num_sub_steps) s.num_sub_steps[c] = 0;
-            if (s.status) s.status[c] = 0;
+            if (s.num_iterations) gst(s.num_iterations + (c), 0);
+            if (s.num_sub_steps) gst(s.num_sub_steps + (c), 0);
+            if (s.status) gst(s.status + (c), 0);
           }
           cell = -1;
         } else {
@@ -547,16 +625,16 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         for (int r = 0; r < R; ++r) {
           const int i = r * L + t;
           if (i < naq) {
-            s.pri_molal[i * ld + c] = x[r];
-            s.total[i * ld + c] = total[r];
+            gst(s.pri_molal + (i * ld + c), x[r]);
+            gst(s.total + (i * ld + c), total[r]);
           } else if (i < N) {
-            s.immobile[(i - naq) * ld + c] = x[r];
+            gst(s.immobile + ((i - naq) * ld + c), x[r]);
           }
         }
         if (t == 0) {
-          if (s.num_iterations) s.num_iterations[c] = nit_total;
-          if (s.num_sub_steps) s.num_sub_steps[c] = nsub;
-          if (s.status) s.status[c] = status;
+          if (s.num_iterations) gst(s.num_iterations + (c), nit_total);
+          if (s.num_sub_steps) gst(s.num_sub_steps + (c), nsub);
+          if (s.status) gst(s.status + (c), status);
         }
         cell = -1;
         continue;
@@ -591,7 +669,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int r = 0; r < R; ++r) {
         const int i = r * L + t;
         if (i < naq) {
-          Ip += x[r] * n.Zp[i] * n.Zp[i];
+          Ip += x[r] * gld(n.Zp + i) * gld(n.Zp + i);
           if (i != n.h2o_aq_id) sp_ += x[r];
         }
       }
@@ -601,11 +679,11 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int r = 0; r < R; ++r) {
         const int i = r * L + t;
         if (i < naq)
-          lngam[r] = fabs(n.Zp[i]) > 1.0e-10 ? log(dh_gamma(n.Zp[i], n.a0p[i], sI, I, n)) : 0.0;
+          lngam[r] = fabs(gld(n.Zp + i)) > 1.0e-10 ? log(dh_gamma(gld(n.Zp + i), gld(n.a0p + i), sI, I, n)) : 0.0;
       }
       for (int k = t; k < nc; k += L)
-        lng[k] = fabs(n.cplx_Z[k]) > 1.0e-10
-                     ? log(dh_gamma(n.cplx_Z[k], n.cplx_a0[k], sI, I, n)) : 0.0;
+        lng[k] = fabs(gld(n.cplx_Z + k)) > 1.0e-10
+                     ? log(dh_gamma(gld(n.cplx_Z + k), gld(n.cplx_a0 + k), sI, I, n)) : 0.0;
       if (n.act_h2o) {
         const double a = 1.0 - 0.017 * team_sum<L>(sp_ + sec_sum);
         ln_aw = a > 0.0 ? log(a) : 0.0;
@@ -627,54 +705,108 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int j = 0; j < NM; ++j) Jt[i * (NM + 1) + j] = (JT)0;
     }
     team_sync();
-    // phase B: complexes (RTotalAqueous) -> totals and Jacobian
+    // phases B and C: complexes (RTotalAqueous) and kinetic minerals
+    // (RKineticMineral) scattered into the LDS tile.  LDS operations of a
+    // wave complete in order, so an LDS read issued after a batch of atomics
+    // waits for all of them, and a dependent chain of network loads inside
+    // the scatter loops would pay the global-memory latency per element.
+    // Each reaction row (ids, stoichiometry) is therefore staged in
+    // registers with one batch of global loads, every LDS read (ln
+    // activities, row scales, mineral state) is issued before the first
+    // atomic of its chunk, and the atomics then run from registers only.
     sec_I = sec_sum = 0.0;
-    for (int k = t; k < nc; k += L) {
-      int ncs;
-      const int* ids;
-      const double* st;
-      const double lnQK = complex_lnqk(n, k, lnact, ln_aw, temp, &ncs, &ids, &st);
-      const double m_k = exp(lnQK - lng[k]);
-      sec_I += m_k * n.cplx_Z[k] * n.cplx_Z[k];
-      sec_sum += m_k;
-      for (int q = 0; q < ncs; ++q) {
-        const int i = ids[1 + q];
-        atomicAdd(&tot[i], st[q] * m_k);
-        if (wet) {
-          const double ri = st[q] * m_k * rsv[i] * psvd * dkl;
-          for (int u = 0; u < ncs; ++u)
-            atomicAdd(&Jt[i * (NM + 1) + ids[1 + u]], (JT)(ri * st[u]));
+    const double jc = psvd * dkl;
+    const int cms = n.cplx_ms, mms = n.mnrl_ms;
+    // minerals m = q*L + t (q < KM): rates and scatter weights (reads only)
+    RxnRow<kRowSlots> mrow[KM];
+    double mv[KM], mg[KM];
+    bool mon[KM];
+#pragma unroll
+    for (int q = 0; q < KM; ++q) {
+      const int m = q * L + t;
+      mv[q] = mg[q] = 0.0;
+      mon[q] = false;
+      if (m < nm) {
+        const double logK = n.isothermal || !n.has_mnrl_logKcoef
+                                ? gld(n.mnrl_logK + m)
+                                : interp_logk(n.mnrl_logKcoef + 5 * m, temp);
+        double lnQK0 = -logK * kLogToLn;
+        if (n.has_mnrl_h2o) lnQK0 += gld(n.mnrl_h2o + m) * ln_aw;
+        const double lnQK =
+            mrow[q].load(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, lnact, rsv, lnQK0);
+        double rt = 0.0;
+        if (area[m] != 0.0) {  // zero-area minerals give Im = 0
+          const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp);
+          if (mr.active) {
+            rt = mr.Im;
+            // dry cells: RReaction adds nothing (reaction.F90:4083), but the
+            // kinetic-state update still uses this rate
+            mon[q] = wet;
+            mv[q] = mr.Im * vol;
+            // log form: d res_i / d ln m_j = nu_i dIm/dQK nu_j QK * rho/1000
+            mg[q] = mr.dIm_dQK * vol * mr.QK * dkl;
+          }
         }
+        rate[m] = rt;
       }
     }
-    // phase C: kinetic minerals (RKineticMineral) -> residual and J
-    const int mms = n.mnrl_ms;
-    for (int m = t; m < nm; m += L) {
-      rate[m] = 0.0;
-      if (area[m] == 0.0) continue;  // zero-area minerals give Im = 0
-      const int* ids = n.mnrl_ids + m * (mms + 1);
-      const double* st = n.mnrl_st + m * mms;
-      const int ncs = ids[0];
-      double logK = n.mnrl_logK[m];
-      if (!n.isothermal && n.has_mnrl_logKcoef) logK = interp_logk(n.mnrl_logKcoef + 5 * m, temp);
-      double lnQK = -logK * kLogToLn;
-      if (n.has_mnrl_h2o) lnQK += n.mnrl_h2o[m] * ln_aw;
-      for (int u = 0; u < ncs; ++u) lnQK += st[u] * lnact[ids[1 + u]];
-      const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp);
-      if (!mr.active) continue;
-      rate[m] = mr.Im;
-      // dry cells: RReaction adds nothing (reaction.F90:4083), but the
-      // kinetic-state update still uses this rate
-      if (!wet) continue;
-      const double Im = mr.Im * vol;
-      for (int u = 0; u < ncs; ++u) atomicAdd(&racc[ids[1 + u]], st[u] * Im);
-      // log form: d res_i / d ln m_j = nu_i dIm/dQK nu_j QK * rho/1000
-      const double g = mr.dIm_dQK * vol * mr.QK * dkl;
-      for (int u = 0; u < ncs; ++u) {
-        const int i = ids[1 + u];
-        const double gi = g * st[u] * rsv[i];
-        for (int v = 0; v < ncs; ++v) atomicAdd(&Jt[i * (NM + 1) + ids[1 + v]], (JT)(gi * st[v]));
+    // complexes k = k0 + q*L + t, KC per lane per chunk
+    for (int k0 = 0; k0 < nc; k0 += KC * L) {
+      RxnRow<kRowSlots> crow[KC];
+      double mk[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int k = k0 + q * L + t;
+        mk[q] = 0.0;
+        if (k < nc) {
+          const double logK = n.isothermal || !n.has_cplx_logKcoef
+                                  ? gld(n.cplx_logK + k)
+                                  : interp_logk(n.cplx_logKcoef + 5 * k, temp);
+          double lnQK = -logK * kLogToLn;
+          if (n.has_cplx_h2o) lnQK += gld(n.cplx_h2o + k) * ln_aw;
+          lnQK = crow[q].load(n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, lnact, rsv, lnQK);
+          mk[q] = exp(lnQK - lng[k]);
+          const double z = gld(n.cplx_Z + k);
+          sec_I += mk[q] * z * z;
+          sec_sum += mk[q];
+        }
       }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int k = k0 + q * L + t;
+        if (k < nc)
+          crow[q].template scatter<NM, JT>(n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, mk[q],
+                                           mk[q] * jc, wet, tot, Jt, rsv);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KM; ++q) {
+      const int m = q * L + t;
+      if (m < nm && mon[q])
+        mrow[q].template scatter<NM, JT>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, mv[q],
+                                         mg[q], true, racc, Jt, rsv);
+    }
+    // minerals beyond the register slots (more than KM*L of them)
+    for (int m = KM * L + t; m < nm; m += L) {
+      double rt = 0.0;
+      if (area[m] != 0.0) {
+        RxnRow<kRowSlots> row;
+        const double logK = n.isothermal || !n.has_mnrl_logKcoef
+                                ? gld(n.mnrl_logK + m)
+                                : interp_logk(n.mnrl_logKcoef + 5 * m, temp);
+        double lnQK = -logK * kLogToLn;
+        if (n.has_mnrl_h2o) lnQK += gld(n.mnrl_h2o + m) * ln_aw;
+        lnQK = row.load(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, lnact, rsv, lnQK);
+        const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp);
+        if (mr.active) {
+          rt = mr.Im;
+          if (wet)
+            row.template scatter<NM, JT>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms,
+                                         mr.Im * vol, mr.dIm_dQK * vol * mr.QK * dkl, true, racc,
+                                         Jt, rsv);
+        }
+      }
+      rate[m] = rt;
     }
     team_sync();
 
@@ -694,7 +826,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           double d = 1.0;
           if (i >= naq)
             for (int q = 0; q < n.ndecay; ++q)
-              if (n.decay_id[q] == i - naq) d += n.decay_k[q] * vol * x[r] * rs[r];
+              if (gld(n.decay_id + q) == i - naq) d += gld(n.decay_k + q) * vol * x[r] * rs[r];
 #pragma unroll
           for (int j = 0; j < NM; ++j)
             if (j == i) a[r][j] += (JT)d;
@@ -724,7 +856,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         res[r] = wet ? (x[r] * vol - fixed[r]) / tdt : 0.0;
         if (wet)
           for (int q = 0; q < n.ndecay; ++q)
-            if (n.decay_id[q] == i - naq) res[r] += n.decay_k[q] * vol * x[r];
+            if (gld(n.decay_id + q) == i - naq) res[r] += gld(n.decay_k + q) * vol * x[r];
       }
       if (wet) res[r] += racc[i];
       mx = fmax(mx, fabs(res[r]));
@@ -768,6 +900,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           if (p < N) {
             const JT pv = bcast<L, ow>(a[sl][p]);
             if (t == ow && !((double)fabs(a[sl][p]) > kPivTol * amax[sl])) bad = 1;
+            const JT pinv = (JT)1 / pv;
             JT pr[NM];
             static_for<p + 1, NM>([&](auto Q) {
               constexpr int j = decltype(Q)::value;
@@ -777,7 +910,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 #pragma unroll
             for (int r = 0; r < R; ++r) {
               if (r * L + t > p) {
-                const JT l = a[r][p] / pv;
+                const JT l = a[r][p] * pinv;
                 a[r][p] = l;
                 static_for<p + 1, NM>([&](auto Q) {
                   constexpr int j = decltype(Q)::value;
@@ -786,6 +919,9 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                 rhs[r] -= l * prhs;
               }
             }
+            // the diagonal of row p is not read again by the elimination:
+            // keep its reciprocal for the back substitution
+            if (t == ow) a[sl][p] = pinv;
           }
         });
         bad = team_or<L>(bad);
@@ -801,7 +937,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                 constexpr int j = decltype(Q)::value;
                 sum -= a[sl][j] * xs[j];
               });
-              xs[p] = bcast<L, ow>(sum / a[sl][p]);
+              xs[p] = bcast<L, ow>(sum * a[sl][p]);
             }
           });
 #pragma unroll
@@ -888,7 +1024,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     if (outcome == kConverged) {
       // RUpdateKineticState: mineral volume fractions with the final rates
       for (int m = t; m < nm; m += L) {
-        vf[m] += rate[m] * n.mnrl_Vm[m] * tdt;
+        vf[m] += rate[m] * gld(n.mnrl_Vm + m) * tdt;
         if (vf[m] < 0.0) vf[m] = 0.0;
       }
       nit_total += nit;
@@ -906,7 +1042,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     } else if (outcome != kContinue && sizeof(JT) == 4 && escalate) {
       // The FP32 factorization could not bring this cell to convergence:
       // leave its inputs untouched and hand it to the FP64 pass.
-      if (t == 0) s.status[c] = kEscalate;
+      if (t == 0) gst(s.status + (c), kEscalate);
       cell = -1;
       continue;
     } else if (outcome != kContinue) {
@@ -936,36 +1072,33 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int r = 0; r < R; ++r) {
         const int i = r * L + t;
         if (i < naq) {
-          s.pri_molal[i * ld + c] = x[r];
-          s.total[i * ld + c] = (ok && small[r]) ? small_val[r] : total[r];
+          gst(s.pri_molal + (i * ld + c), x[r]);
+          gst(s.total + (i * ld + c), (ok && small[r]) ? small_val[r] : total[r]);
           if (n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER)
-            s.pri_act_coef[i * ld + c] = exp(lngam[r]);
+            gst(s.pri_act_coef + (i * ld + c), exp(lngam[r]));
         } else if (i < N) {
-          s.immobile[(i - naq) * ld + c] = (ok && small[r]) ? small_val[r] : x[r];
+          gst(s.immobile + ((i - naq) * ld + c), (ok && small[r]) ? small_val[r] : x[r]);
         }
       }
       // secondary molalities at the final free molalities (the last
       // RTAuxVarCompute, reaction.F90:4052): recomputed here instead of
       // being kept per complex in LDS
       for (int k = t; k < nc; k += L) {
-        int ncs;
-        const int* ids;
-        const double* st;
-        const double lnQK = complex_lnqk(n, k, lnact, ln_aw, temp, &ncs, &ids, &st);
-        s.sec_molal[k * ld + c] = exp(lnQK - lng[k]);
+        const double lnQK = complex_lnqk(n, k, lnact, ln_aw, temp);
+        gst(s.sec_molal + (k * ld + c), exp(lnQK - lng[k]));
         if (n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER)
-          s.sec_act_coef[k * ld + c] = exp(lng[k]);
+          gst(s.sec_act_coef + (k * ld + c), exp(lng[k]));
       }
       for (int m = t; m < nm; m += L) {
-        s.mnrl_volfrac[m * ld + c] = vf[m];
-        if (s.mnrl_rate) s.mnrl_rate[m * ld + c] = rate[m];
+        gst(s.mnrl_volfrac + (m * ld + c), vf[m]);
+        if (s.mnrl_rate) gst(s.mnrl_rate + (m * ld + c), rate[m]);
       }
       if (t == 0) {
         if (s.ln_act_h2o && n.act_h2o && n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER)
-          s.ln_act_h2o[c] = ln_aw;
-        if (s.num_iterations) s.num_iterations[c] = nit_total;
-        if (s.num_sub_steps) s.num_sub_steps[c] = nsub;
-        if (s.status) s.status[c] = status;
+          gst(s.ln_act_h2o + (c), ln_aw);
+        if (s.num_iterations) gst(s.num_iterations + (c), nit_total);
+        if (s.num_sub_steps) gst(s.num_sub_steps + (c), nsub);
+        if (s.status) gst(s.status + (c), status);
       }
       team_sync();
       cell = -1;

@@ -50,8 +50,18 @@ def load(path: str = LIB_PATH):
     L.pfchem_residual.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.pfchem_set_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_get_launch.argtypes = [vp, C.c_int, vp]
+    L.pfchem_tangent.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, vp, C.c_int, vp, vp, vp,
+                                 vp]
+    L.pfchem_adjoint.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, C.c_int, vp, vp, vp, vp,
+                                 vp]
     _lib = L
     return L
+
+
+def _stream_ptr(stream):
+    """hipStream_t of a torch stream (torch's current stream by default)"""
+    import torch
+    return (stream or torch.cuda.current_stream()).cuda_stream
 
 
 def abi_strerror(code):
@@ -165,6 +175,54 @@ class Context:
         not fit this network"""
         c = abi.Launch(lanes_per_cell, block_size, blocks_per_cu, rows_per_lane)
         return self.L.pfchem_set_launch(self.h, variant, C.byref(c)) == abi.PFCHEM_OK
+
+    def _weights(self, w_vf, w_sec):
+        nm, nc = self.net.nkinmnrl, self.net.neqcplx
+        wv = np.ascontiguousarray(np.zeros(nm) if w_vf is None else w_vf, dtype=np.float64)
+        ws = np.ascontiguousarray(np.zeros(nc) if w_sec is None else w_sec, dtype=np.float64)
+        if wv.shape != (nm,) or ws.shape != (nc,):
+            raise ValueError("weights must have nkinmnrl / neqcplx entries")
+        return wv, ws
+
+    def tangent(self, st: DeviceState, dt: float, dp, w_vf=None, w_sec=None, exact=True,
+                stream=None):
+        """pfchem_tangent: (df per cell, f per cell, status per cell) as device
+        tensors, for the parameter direction dp (log10 k then log10 K)"""
+        import torch
+        wv, ws = self._weights(w_vf, w_sec)
+        dpv = np.ascontiguousarray(dp, dtype=np.float64)
+        if dpv.shape != (wv.size + ws.size,):
+            raise ValueError("dp must have nkinmnrl + neqcplx entries")
+        n = st.ncell
+        df = torch.zeros(n, dtype=torch.float64, device="cuda")
+        f = torch.zeros(n, dtype=torch.float64, device="cuda")
+        status = torch.zeros(n, dtype=torch.int32, device="cuda")
+        s = st.to_abi()
+        self._check(self.L.pfchem_tangent(self.h, n, C.byref(s), dt, wv.ctypes.data,
+                                          ws.ctypes.data, dpv.ctypes.data, 1 if exact else 0,
+                                          df.data_ptr(), f.data_ptr(), status.data_ptr(),
+                                          _stream_ptr(stream)), "pfchem_tangent")
+        return df, f, status
+
+    def adjoint(self, st: DeviceState, dt: float, w_vf=None, w_sec=None, exact=True,
+                per_cell=False, stream=None):
+        """pfchem_adjoint: (gradient summed over cells (numpy), f per cell,
+        status per cell, per-cell gradients or None)"""
+        import torch
+        wv, ws = self._weights(w_vf, w_sec)
+        n = st.ncell
+        np_ = wv.size + ws.size
+        grad = np.zeros(max(np_, 1))
+        f = torch.zeros(n, dtype=torch.float64, device="cuda")
+        status = torch.zeros(n, dtype=torch.int32, device="cuda")
+        gc = torch.zeros((np_, n), dtype=torch.float64, device="cuda") if per_cell else None
+        s = st.to_abi()
+        self._check(self.L.pfchem_adjoint(self.h, n, C.byref(s), dt, wv.ctypes.data,
+                                          ws.ctypes.data, 1 if exact else 0, grad.ctypes.data,
+                                          gc.data_ptr() if gc is not None else None,
+                                          f.data_ptr(), status.data_ptr(),
+                                          _stream_ptr(stream)), "pfchem_adjoint")
+        return grad[:np_], f, status, gc
 
     def get_launch(self, variant):
         c = abi.Launch()
