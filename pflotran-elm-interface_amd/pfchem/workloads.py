@@ -332,3 +332,84 @@ def random_network_workload(ncell: int = 1 << 20, naq: int = 15, ncplx: int = 60
              "U{0..5} per cell")
     return Workload("random_n%d_c%d_m%d" % (naq, ncplx, nmnrl), net, bench_options(), st,
                     dt, seed, notes)
+
+
+# ---------------------------------------------------------------------------
+# SomDec decomposition networks (ELM-CN litter / SOM cascade)
+# ---------------------------------------------------------------------------
+SOMDEC_PRIMARIES = ["CO2(aq)", "NH4+", "NO3-", "N2O(aq)", "H+"]
+SOMDEC_IMMOBILE = ["SOM1", "SOM2", "SOM3", "SOM4", "Lit1C", "Lit2C", "Lit3C",
+                   "Lit1N", "Lit2N", "Lit3N"]
+
+
+def somdec_network(rates=None, som_cn=(12.0, 12.0, 10.0, 10.0), half_saturation=1.0e-4,
+                   temperature_function=0, Q10=2.0) -> ReactionNetwork:
+    """The CLM-CN cascade of the clm_cn3 regression deck
+    (regression_tests/ngee/CLMCNplus/clm_cn3.in): litter pools Lit1-3 with
+    variable C:N, SOM1-4 with fixed C:N, Monod NH4+/NO3- limitation on the
+    litter reactions.  rates: 7 first-order rate constants (1/s) in the
+    deck's reaction order, default the deck's turnover times."""
+    from . import somdec as SD
+    h, d, y = 3600.0, 86400.0, 365.0 * 86400.0
+    turnover = [20 * h, 14 * d, 71 * d, 14 * d, 71 * d, 2 * y, 27.4 * y]
+    if rates is None:
+        rates = [1.0 / t for t in turnover]
+    pools = [SD.Pool("SOM%d" % (i + 1), som_cn[i]) for i in range(4)] + \
+        [SD.Pool("Lit1"), SD.Pool("Lit2"), SD.Pool("Lit3")]
+    mon = [SD.Monod("NH4+", half_saturation), SD.Monod("NO3-", half_saturation)]
+    cascade = [("Lit1", [("SOM1", 0.61)], mon), ("Lit2", [("SOM2", 0.45)], mon),
+               ("Lit3", [("SOM3", 0.71)], mon), ("SOM1", [("SOM2", 0.72)], []),
+               ("SOM2", [("SOM3", 0.54)], []), ("SOM3", [("SOM4", 0.45)], []),
+               ("SOM4", [], [])]
+    rx = [SD.Reaction(up, down, rate_constant=k, monod=list(m),
+                      temperature_response_function=temperature_function, Q10=Q10)
+          for (up, down, m), k in zip(cascade, rates)]
+    sd = SD.build_somdec(pools, rx, SOMDEC_PRIMARIES, SOMDEC_IMMOBILE)
+    return ReactionNetwork(primary_species_names=list(SOMDEC_PRIMARIES),
+                           primary_spec_Z=np.zeros(len(SOMDEC_PRIMARIES)),
+                           primary_spec_a0=np.zeros(len(SOMDEC_PRIMARIES)),
+                           immobile_names=list(SOMDEC_IMMOBILE), somdec=sd)
+
+
+def somdec_options(**kw) -> SolverOptions:
+    return bench_options(act_coef_update_frequency=abi.ACT_COEF_FREQUENCY_OFF, **kw)
+
+
+def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float = 1800.0,
+                    ld: Optional[int] = None) -> Workload:
+    """ELM-CN decomposition columns (500,000 columns x 15 layers by default):
+    per-reaction first-order rates drawn U[1e-8, 1e-5] /s, SOM C:N (mass)
+    U[8, 30], Monod half-saturation 1e-6 mol/m^3; per cell a temperature
+    U[0, 35] C acting through Q10 = 2 and a moisture scalar f_w U[0, 1]
+    (the ELM w_scalar input); litter C (mol/m^3) logU[1e-2, 1] with C:N
+    (mass) U[20, 80], SOM C logU[1e-3, 1], NH4+ / NO3- logU[1e-7, 1e-4]
+    mol/L, CO2 logU[1e-6, 1e-3] mol/L, N2O 1e-20 mol/L, pH 6."""
+    rng = _rng(seed)
+    rates = 10.0 ** rng.uniform(-8.0, -5.0, 7)
+    som_cn = rng.uniform(8.0, 30.0, 4)
+    net = somdec_network(rates=list(rates), som_cn=tuple(som_cn), half_saturation=1.0e-6,
+                         temperature_function=2, Q10=2.0)
+    st = CellState(net, ncell, ld, somdec_scalars=True)
+    n = ncell
+    lu = lambda lo, hi, k: 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), k)
+    T = np.vstack([lu(1e-6, 1e-3, n), lu(1e-7, 1e-4, n), lu(1e-7, 1e-4, n),
+                   np.full(n, 1e-20), np.full(n, 1e-6)])
+    st["total"][:, :n] = T
+    st["pri_molal"][:, :n] = T          # no complexes, unit density below
+    st["pri_act_coef"][:, :n] = 1.0
+    litc = np.vstack([lu(1e-2, 1.0, n) for _ in range(3)])
+    litcn = rng.uniform(20.0, 80.0, (3, n))
+    from .somdec import CN_RATIO_MASS_TO_MOL
+    im = np.vstack([lu(1e-3, 1.0, n) for _ in range(4)] + [litc] +
+                   [litc / litcn / CN_RATIO_MASS_TO_MOL])
+    st["immobile"][:, :n] = im
+    st["porosity"][0, :n] = rng.uniform(0.3, 0.6, n)
+    st["saturation"][0, :n] = 1.0
+    st["den_kg"][0, :n] = 1000.0
+    st["volume"][0, :n] = 1.0
+    st["temperature"][0, :n] = rng.uniform(0.0, 35.0, n)
+    st["somdec_f_t"][0, :n] = 1.0
+    st["somdec_f_w"][0, :n] = rng.uniform(0.0, 1.0, n)
+    notes = ("SomDec ELM-CN cascade (clm_cn3 structure); rates U[1e-8,1e-5]/s per reaction, "
+             "SOM C:N U[8,30], Q10 = 2 with per-cell temperature, per-cell f_w U[0,1]")
+    return Workload("somdec_%d" % ncell, net, somdec_options(), st, dt, seed, notes)

@@ -46,9 +46,28 @@ def parse_gold(path):
     return out
 
 
+def parse_obs(path, nrows):
+    """header and the first nrows rows of a *-obs-0.tec.gold time series"""
+    with open(path) as f:
+        header = [h.strip().strip('"') for h in f.readline().split('",')]
+        rows = []
+        for line in f:
+            if len(rows) == nrows:
+                break
+            rows.append([float(v) for v in line.split()])
+    return header, rows
+
+
 def main(ref):
     here = os.path.dirname(os.path.abspath(__file__))
     golds = {}
+    # SomDec cascade deck: the first steps of the observation time series
+    # (7 significant digits) and the end-of-run regression values
+    deck = "regression_tests/ngee/CLMCNplus/clm_cn3"
+    header, rows = parse_obs(os.path.join(ref, deck + "-obs-0.tec.gold"), 25)
+    golds["clm_cn3"] = {"source": deck + "-obs-0.tec.gold",
+                        "columns": header, "rows": rows,
+                        "final": parse_gold(os.path.join(ref, deck + ".regression.gold"))}
     for case, (gold, db) in CASES.items():
         golds[case] = {"source": gold, "database": os.path.basename(db),
                        "values": parse_gold(os.path.join(ref, gold))}
