@@ -19,7 +19,9 @@
  * rate / d NH4 columns without the theta*1000 factor of the concentration
  * conversion, and dnh4_duc = dco2_duc * mineral_n_stoich in React1): the
  * Jacobian steers the Newton iteration only, not its solution.
- * Two intended differences: the variable C:N stoichiometry lives in the
+ * Downstream entries without a species (id < 0) are skipped in the
+ * Jacobian columns of NH4+ / NO3- too (the reference checks them only in
+ * the residual).  Two intended differences: the variable C:N stoichiometry lives in the
  * cell (orc_cell.sd_*), starting from the Setup values at each step,
  * instead of in the shared sandbox object (:1817-1858), so results do not
  * depend on cell order; and a reaction skipped by its abiotic factors
@@ -417,7 +419,7 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
     for (int j = 0; j < sd->n_downstream_pools[r]; ++j) {
       const int dc = sd->downstream_c_id[r * MD + j];
       const double ddc = sd->downstream_stoich[r * MD + j] * (-1.0 * duc_dnh4);
-      J(x, imm(x, dc), rnh4, -ddc);
+      if (dc >= 0) J(x, imm(x, dc), rnh4, -ddc);
     }
     J(x, rnh4, rnh4, -dnh4_dnh4 * dtot(x, rnh4, rnh4));
     if (sd->upstream_nimm_id[r] >= 0) J(x, imm(x, sd->upstream_nimm_id[r]), rnh4, dnh4_dnh4);
@@ -433,7 +435,7 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
       const int dn = sd->downstream_n_id[r * MD + j];
       const double ddn =
           sd->downstream_stoich[r * MD + j] * (-1.0 * duc_dnh4) * c->sd_downstream_nc[r * MD + j];
-      J(x, imm(x, dn), rnh4, -ddn);
+      if (dn >= 0) J(x, imm(x, dn), rnh4, -ddn);
     }
   }
   if (has_no3) {
@@ -444,7 +446,7 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
     for (int j = 0; j < sd->n_downstream_pools[r]; ++j) {
       const int dc = sd->downstream_c_id[r * MD + j];
       const double ddc = sd->downstream_stoich[r * MD + j] * (-1.0 * duc_dno3);
-      J(x, imm(x, dc), rno3, -ddc);
+      if (dc >= 0) J(x, imm(x, dc), rno3, -ddc);
     }
     if (has_nh4) {
       J(x, rnh4, rno3, -dnh4_dno3 * dtot(x, rnh4, rno3));
@@ -460,7 +462,7 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
       const int dn = sd->downstream_n_id[r * MD + j];
       const double ddn =
           sd->downstream_stoich[r * MD + j] * (-1.0 * duc_dno3) * c->sd_downstream_nc[r * MD + j];
-      J(x, imm(x, dn), rno3, -ddn);
+      if (dn >= 0) J(x, imm(x, dn), rno3, -ddn);
     }
   }
   return nimm;

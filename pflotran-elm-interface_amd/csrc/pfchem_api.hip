@@ -71,6 +71,89 @@ const size_t kNone = ~size_t(0);
 // is rewritten (stream-ordered) only after kStateRing further launches
 const int kStateRing = 256;
 
+// SomDec descriptor: ids in range; the kernels take totals as free molality
+// times density (no complexes) and do not carry the Monod oxygen response
+int check_somdec(const pfchem_network* net, std::string& msg) {
+  const pfchem_somdec* sd = net->somdec;
+  const int naq = net->naqcomp, nimm = net->nimmobile, nr = sd->nrxn;
+  auto aq = [&](int id, bool req) { return req ? (id >= 0 && id < naq) : (id >= -1 && id < naq); };
+  auto im = [&](int id, bool req) { return req ? (id >= 0 && id < nimm) : (id >= -1 && id < nimm); };
+  if (nr < 0 || nr > 32) {
+    msg = "somdec: 0..32 reactions";
+    return PFCHEM_ERR_UNSUPPORTED;
+  }
+  if (net->neqcplx > 0) {
+    msg = "somdec: networks with aqueous complexes are not supported by the kernels";
+    return PFCHEM_ERR_UNSUPPORTED;
+  }
+  if (naq + nimm > 32) {
+    msg = "somdec: at most 32 primary + immobile species";
+    return PFCHEM_ERR_UNSUPPORTED;
+  }
+  if (!aq(sd->species_id_co2, true) || !aq(sd->species_id_nh4, true) ||
+      !aq(sd->species_id_no3, false) || !aq(sd->species_id_n2o, false) ||
+      !aq(sd->species_id_proton, false) || !im(sd->species_id_hr, false) ||
+      !im(sd->species_id_nmin, false) || !im(sd->species_id_nimm, false) ||
+      !im(sd->species_id_nimp, false) || !im(sd->species_id_ngasmin, false)) {
+    msg = "somdec: species id out of range";
+    return PFCHEM_ERR_INVALID;
+  }
+  if (nr > 0 && (!sd->rate_constant || !sd->rate_decomposition || !sd->rate_ad_factor ||
+                 !sd->upstream_c_id || !sd->upstream_n_id || !sd->upstream_hr_id ||
+                 !sd->upstream_nmin_id || !sd->upstream_nimp_id || !sd->upstream_nimm_id ||
+                 !sd->upstream_nc || !sd->upstream_is_varycn || !sd->n_downstream_pools ||
+                 !sd->downstream_c_id || !sd->downstream_n_id || !sd->downstream_stoich ||
+                 !sd->downstream_nc || !sd->downstream_is_varycn || !sd->mineral_c_stoich ||
+                 !sd->mineral_n_stoich || !sd->temperature_response_function || !sd->Q10 ||
+                 !sd->Ea || !sd->moisture_response_function || !sd->Ox_response_function ||
+                 !sd->Ox_half_saturation || !sd->n_monod || !sd->monod_specid ||
+                 !sd->monod_half_saturation || !sd->monod_threshold ||
+                 !sd->monod_pool_normalized || !sd->n_inhibition || !sd->inhibition_specid ||
+                 !sd->inhibition_type || !sd->inhibition_constant || !sd->inhibition_constant2)) {
+    msg = "somdec: missing per-reaction array";
+    return PFCHEM_ERR_INVALID;
+  }
+  for (int r = 0; r < nr; ++r) {
+    if (!im(sd->upstream_c_id[r], true) || !im(sd->upstream_n_id[r], false) ||
+        !im(sd->upstream_hr_id[r], false) || !im(sd->upstream_nmin_id[r], false) ||
+        !im(sd->upstream_nimp_id[r], false) || !im(sd->upstream_nimm_id[r], false) ||
+        (sd->upstream_is_varycn[r] && sd->upstream_n_id[r] < 0) ||
+        sd->n_downstream_pools[r] < 0 || sd->n_downstream_pools[r] > PFCHEM_SOMDEC_MAX_DOWN ||
+        sd->n_monod[r] < 0 || sd->n_monod[r] > PFCHEM_SOMDEC_MAX_MONOD ||
+        sd->n_inhibition[r] < 0 || sd->n_inhibition[r] > PFCHEM_SOMDEC_MAX_INHIB) {
+      msg = "somdec: bad reaction " + std::to_string(r);
+      return PFCHEM_ERR_INVALID;
+    }
+    for (int j = 0; j < sd->n_downstream_pools[r]; ++j) {
+      const int k = r * PFCHEM_SOMDEC_MAX_DOWN + j;
+      if (!im(sd->downstream_c_id[k], false) || !im(sd->downstream_n_id[k], false) ||
+          (sd->downstream_is_varycn[k] && sd->downstream_n_id[k] < 0)) {
+        msg = "somdec: bad downstream pool in reaction " + std::to_string(r);
+        return PFCHEM_ERR_INVALID;
+      }
+    }
+    for (int q = 0; q < sd->n_monod[r]; ++q) {
+      const int id = sd->monod_specid[r * PFCHEM_SOMDEC_MAX_MONOD + q];
+      if (id >= naq || id < -nimm) {
+        msg = "somdec: bad Monod species in reaction " + std::to_string(r);
+        return PFCHEM_ERR_INVALID;
+      }
+    }
+    for (int q = 0; q < sd->n_inhibition[r]; ++q) {
+      const int id = sd->inhibition_specid[r * PFCHEM_SOMDEC_MAX_INHIB + q];
+      if (id >= naq || id < -nimm) {
+        msg = "somdec: bad inhibition species in reaction " + std::to_string(r);
+        return PFCHEM_ERR_INVALID;
+      }
+    }
+    if (sd->Ox_response_function[r] == 1) {
+      msg = "somdec: the Monod oxygen response is not supported";
+      return PFCHEM_ERR_UNSUPPORTED;
+    }
+  }
+  return 0;
+}
+
 int check_reactions(const int32_t* ids, int n, int maxsp, int naq, const char* what,
                     std::string& msg) {
   for (int k = 0; k < n; ++k) {
@@ -143,10 +226,16 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
       delete ctx;
       return PFCHEM_ERR_INVALID;
     }
-  if (opt->act_coef_update_algorithm == PFCHEM_ACT_COEF_ALGORITHM_NEWTON ||
-      net->somdec != nullptr) {
+  if (opt->act_coef_update_algorithm == PFCHEM_ACT_COEF_ALGORITHM_NEWTON) {
     delete ctx;
     return PFCHEM_ERR_UNSUPPORTED;
+  }
+  if (net->somdec) {
+    const int rc = check_somdec(net, msg);
+    if (rc) {
+      delete ctx;
+      return rc;
+    }
   }
   if (opt->max_dlnC_rreact <= 0.0 || opt->maximum_reaction_iterations < 1 ||
       opt->maximum_reaction_cuts < 0) {
@@ -191,6 +280,50 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   size_t omt = addd(net->kinmnrl_affinity_threshold, nm), oml = addd(net->kinmnrl_rate_limiter, nm);
   size_t omI = addi(net->kinmnrl_irreversible, nm);
   size_t odi = addi(net->decayspecid, net->ndecay_rxn), odk = addd(net->decay_rate_constant, net->ndecay_rxn);
+  // SomDec per-reaction arrays
+  const pfchem_somdec* sdh = net->somdec;
+  const int nr = sdh ? sdh->nrxn : 0;
+  const size_t nrd = (size_t)nr * PFCHEM_SOMDEC_MAX_DOWN, nrm = (size_t)nr * PFCHEM_SOMDEC_MAX_MONOD;
+  const size_t nri = (size_t)nr * PFCHEM_SOMDEC_MAX_INHIB;
+  size_t so[37];
+  if (sdh) {
+    int q = 0;
+    so[q++] = addd(sdh->rate_constant, nr);
+    so[q++] = addd(sdh->rate_decomposition, nr);
+    so[q++] = addd(sdh->rate_ad_factor, nr);
+    so[q++] = addi(sdh->upstream_c_id, nr);
+    so[q++] = addi(sdh->upstream_n_id, nr);
+    so[q++] = addi(sdh->upstream_hr_id, nr);
+    so[q++] = addi(sdh->upstream_nmin_id, nr);
+    so[q++] = addi(sdh->upstream_nimp_id, nr);
+    so[q++] = addi(sdh->upstream_nimm_id, nr);
+    so[q++] = addd(sdh->upstream_nc, nr);
+    so[q++] = addi(sdh->upstream_is_varycn, nr);
+    so[q++] = addi(sdh->n_downstream_pools, nr);
+    so[q++] = addi(sdh->downstream_c_id, nrd);
+    so[q++] = addi(sdh->downstream_n_id, nrd);
+    so[q++] = addd(sdh->downstream_stoich, nrd);
+    so[q++] = addd(sdh->downstream_nc, nrd);
+    so[q++] = addi(sdh->downstream_is_varycn, nrd);
+    so[q++] = addd(sdh->mineral_c_stoich, nr);
+    so[q++] = addd(sdh->mineral_n_stoich, nr);
+    so[q++] = addi(sdh->temperature_response_function, nr);
+    so[q++] = addd(sdh->Q10, nr);
+    so[q++] = addd(sdh->Ea, nr);
+    so[q++] = addi(sdh->moisture_response_function, nr);
+    so[q++] = addi(sdh->Ox_response_function, nr);
+    so[q++] = addd(sdh->Ox_half_saturation, nr);
+    so[q++] = addi(sdh->n_monod, nr);
+    so[q++] = addi(sdh->monod_specid, nrm);
+    so[q++] = addd(sdh->monod_half_saturation, nrm);
+    so[q++] = addd(sdh->monod_threshold, nrm);
+    so[q++] = addi(sdh->monod_pool_normalized, nrm);
+    so[q++] = addi(sdh->n_inhibition, nr);
+    so[q++] = addi(sdh->inhibition_specid, nri);
+    so[q++] = addi(sdh->inhibition_type, nri);
+    so[q++] = addd(sdh->inhibition_constant, nri);
+    so[q++] = addd(sdh->inhibition_constant2, nri);
+  }
 
   e = hipMalloc(&ctx->dbuf, P.bytes.size());
   if (e != hipSuccess) {
@@ -270,13 +403,69 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   d.mnrl_irrev = pi(omI);
   d.decay_id = pi(odi);
   d.decay_k = pd(odk);
+  d.has_somdec = sdh != nullptr;
+  if (sdh) {
+    DevSomdec& sd = d.sd;
+    sd.nrxn = nr;
+    sd.co2 = sdh->species_id_co2;
+    sd.nh4 = sdh->species_id_nh4;
+    sd.no3 = sdh->species_id_no3;
+    sd.n2o = sdh->species_id_n2o;
+    sd.proton = sdh->species_id_proton;
+    sd.hr = sdh->species_id_hr;
+    sd.nmin = sdh->species_id_nmin;
+    sd.nimm = sdh->species_id_nimm;
+    sd.nimp = sdh->species_id_nimp;
+    sd.ngasmin = sdh->species_id_ngasmin;
+    sd.x0eps = sdh->x0eps;
+    sd.n2o_frac = sdh->n2o_frac_mineralization;
+    sd.inhib_nh4_no3 = sdh->inhibition_nh4_no3;
+    int q = 0;
+    sd.rate_constant = pd(so[q++]);
+    sd.rate_decomposition = pd(so[q++]);
+    sd.rate_ad_factor = pd(so[q++]);
+    sd.upstream_c_id = pi(so[q++]);
+    sd.upstream_n_id = pi(so[q++]);
+    sd.upstream_hr_id = pi(so[q++]);
+    sd.upstream_nmin_id = pi(so[q++]);
+    sd.upstream_nimp_id = pi(so[q++]);
+    sd.upstream_nimm_id = pi(so[q++]);
+    sd.upstream_nc = pd(so[q++]);
+    sd.upstream_is_varycn = pi(so[q++]);
+    sd.n_downstream_pools = pi(so[q++]);
+    sd.downstream_c_id = pi(so[q++]);
+    sd.downstream_n_id = pi(so[q++]);
+    sd.downstream_stoich = pd(so[q++]);
+    sd.downstream_nc = pd(so[q++]);
+    sd.downstream_is_varycn = pi(so[q++]);
+    sd.mineral_c_stoich = pd(so[q++]);
+    sd.mineral_n_stoich = pd(so[q++]);
+    sd.temperature_response_function = pi(so[q++]);
+    sd.Q10 = pd(so[q++]);
+    sd.Ea = pd(so[q++]);
+    sd.moisture_response_function = pi(so[q++]);
+    sd.Ox_response_function = pi(so[q++]);
+    sd.Ox_half_saturation = pd(so[q++]);
+    sd.n_monod = pi(so[q++]);
+    sd.monod_specid = pi(so[q++]);
+    sd.monod_half_saturation = pd(so[q++]);
+    sd.monod_threshold = pd(so[q++]);
+    sd.monod_pool_normalized = pi(so[q++]);
+    sd.n_inhibition = pi(so[q++]);
+    sd.inhibition_specid = pi(so[q++]);
+    sd.inhibition_type = pi(so[q++]);
+    sd.inhibition_constant = pd(so[q++]);
+    sd.inhibition_constant2 = pd(so[q++]);
+  }
   ctx->net = *net;
   ctx->opt = *opt;
   ctx->has_somdec = net->somdec != nullptr;
   // default team shape (lanes per cell x rows per lane) by system size;
   // the autotuner replaces it with a measured choice
-  int L = 16, R = 2;
-  if (d.ncomp <= 4) {
+  int L = 32, R = 1;
+  if (d.has_somdec) {
+    if (d.ncomp <= 16) L = 16;  // SomDec kernels exist for 16x1, 8x2, 32x1
+  } else if (d.ncomp <= 4) {
     L = 4;
     R = 1;
   } else if (d.ncomp <= 8) {
@@ -373,7 +562,13 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
   if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
   hipStream_t s = (hipStream_t)stream;
   if (variant == PFCHEM_KERNEL_PLAIN_FP64) {
-    e = launch_plain_step(ctx->dnet, d, ncell, dt, s);
+    // the one-cell-per-thread SomDec kernel faulted on MI355X without a
+    // cause found in its code (its index checks pass on a host build); the
+    // batched variants run the sandbox
+    if (ctx->dnet.has_somdec)
+      return fail(ctx, PFCHEM_ERR_UNSUPPORTED,
+                  "the plain kernel does not run the SomDec sandbox; use a batched variant");
+    e = launch_plain_step(ctx->dnet, ctx->dnet_dev, d, ncell, dt, s);
   } else {
     const bool mixed = variant == PFCHEM_KERNEL_BATCHED_MIXED;
     const LaunchCfg& cfg = ctx->launch[variant];
@@ -400,7 +595,7 @@ int pfchem_update_activity(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* s
   if (rc) return rc;
   if (ncell == 0) return PFCHEM_OK;
   hipError_t e = hipSetDevice(ctx->device);
-  if (e == hipSuccess) e = launch_activity_lag(ctx->dnet, d, ncell, (hipStream_t)stream);
+  if (e == hipSuccess) e = launch_activity_lag(ctx->dnet, ctx->dnet_dev, d, ncell, (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "activity launch");
   return PFCHEM_OK;
 }

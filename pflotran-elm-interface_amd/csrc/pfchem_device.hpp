@@ -21,6 +21,47 @@ constexpr int kMaxRReactAttempts = 10000;
 
 enum FailReason : int { kOk = 0, kMaxIter = 1, kSingular = 2, kActivity = 3 };
 
+// SomDec sandbox descriptor as kernels see it (pfchem_somdec, 0-based ids)
+struct DevSomdec {
+  int nrxn, co2, nh4, no3, n2o, proton, hr, nmin, nimm, nimp, ngasmin;
+  double x0eps, n2o_frac, inhib_nh4_no3;
+  const double* rate_constant;
+  const double* rate_decomposition;
+  const double* rate_ad_factor;
+  const int* upstream_c_id;
+  const int* upstream_n_id;
+  const int* upstream_hr_id;
+  const int* upstream_nmin_id;
+  const int* upstream_nimp_id;
+  const int* upstream_nimm_id;
+  const double* upstream_nc;
+  const int* upstream_is_varycn;
+  const int* n_downstream_pools;
+  const int* downstream_c_id;
+  const int* downstream_n_id;
+  const double* downstream_stoich;
+  const double* downstream_nc;
+  const int* downstream_is_varycn;
+  const double* mineral_c_stoich;
+  const double* mineral_n_stoich;
+  const int* temperature_response_function;
+  const double* Q10;
+  const double* Ea;
+  const int* moisture_response_function;
+  const int* Ox_response_function;
+  const double* Ox_half_saturation;
+  const int* n_monod;
+  const int* monod_specid;
+  const double* monod_half_saturation;
+  const double* monod_threshold;
+  const int* monod_pool_normalized;
+  const int* n_inhibition;
+  const int* inhibition_specid;
+  const int* inhibition_type;
+  const double* inhibition_constant;
+  const double* inhibition_constant2;
+};
+
 // Network + options as kernels see them: pointers into one device buffer,
 // passed by value as a kernel argument.  Field meanings follow
 // pfchem_network / pfchem_options (include/pfchem.h).
@@ -66,6 +107,9 @@ struct DevNet {
   // immobile decay
   const int* decay_id;
   const double* decay_k;
+  // SomDec sandbox
+  int has_somdec;
+  DevSomdec sd;
 };
 
 // SoA per-cell state, device pointers (pfchem_state)

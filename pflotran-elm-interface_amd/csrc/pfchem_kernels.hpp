@@ -17,15 +17,19 @@ struct LaunchCfg {
 };
 
 int plain_max_ncomp();
-hipError_t launch_plain_step(const DevNet& n, const DevState& s, int64_t ncell,
+hipError_t launch_plain_step(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                              double dt, hipStream_t stream);
-hipError_t launch_activity_lag(const DevNet& n, const DevState& s, int64_t ncell,
+hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                                hipStream_t stream);
 
 // batched (team) kernel: `cursors` holds one 64-bit work cursor per XCD
 // (128-byte stride) that the launcher zeroes with a fill kernel (no memset
 // node) before each launch.
 int team_max_ncomp();
+// team shapes (lanes per cell, rows per lane) with a SomDec instantiation
+constexpr bool team_has_somdec(int L, int R) {
+  return (L == 16 && R == 1) || (L == 8 && R == 2) || (L == 32 && R == 1);
+}
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
 // dn / ds: device copies of the network / state descriptors.  The mixed
 // variant runs the FP32 pass and then an FP64 pass over the cells the FP32

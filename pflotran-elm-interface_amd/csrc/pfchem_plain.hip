@@ -8,6 +8,7 @@
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
 #include "pfchem_lu.hpp"
+#include "pfchem_somdec.hpp"
 
 namespace pfchem {
 namespace {
@@ -178,12 +179,56 @@ __device__ void plain_activity_lag(const DevNet& n, const DevState& s, int64_t c
   }
 }
 
+// Per-cell SomDec stoichiometry of one step: upstream N:C, mineralization C
+// and N coefficients, downstream N:C of every reaction (reset from the
+// descriptor's Setup values at the start of the step).
+struct PlainSdState {
+  double unc[kSdMaxRxn], mcs[kSdMaxRxn], mns[kSdMaxRxn], dnc[kSdMaxRxn * kSdDown];
+  double f_t, f_w;
+};
+
+// SomDec accumulator over the thread's own residual and linear Jacobian
+// (d res / d m for aqueous, d res / d c for immobile columns).  The SomDec
+// networks carry no complexes, so d total_i / d m_j = rho_w / 1000 on the
+// diagonal only.
+template <int MAXN>
+struct PlainSdEnv {
+  PlainCell<MAXN>& w;
+  PlainSdState& sd;
+  int naq, N;
+  double dkl, theta, vol, sat, temp, f_t, f_w;
+  bool has_f_t, has_f_w;
+  __device__ double total(int i) const { return w.total[i]; }
+  __device__ double imm(int i) const { return w.imm[i]; }
+  __device__ double lnact(int i) const { return log(w.x[i]) + log(w.gp[i]); }
+  __device__ double dtot(int i, int j) const { return i == j ? dkl : 0.0; }
+  __device__ void R(int row, double v) { w.res[row] += v; }
+  __device__ void J(int row, int col, double v) { w.J[row * N + col] += v; }
+  __device__ double& unc(int r) { return sd.unc[r]; }
+  __device__ double& mcs(int r) { return sd.mcs[r]; }
+  __device__ double& mns(int r) { return sd.mns[r]; }
+  __device__ double& dnc(int r, int j) { return sd.dnc[r * kSdDown + j]; }
+};
+
+// SomDecReact (reaction_sandbox_somdec.F90:1504-1910) then SomDecNemission
+// (:3477-3640) on the thread's cell
+template <int MAXN>
+__device__ void plain_somdec(const DevNet& n, const DevState& s, PlainCell<MAXN>& w,
+                             PlainSdState& sd, double dt) {
+  PlainSdEnv<MAXN> e{w,     sd,     n.naq, n.ncomp, w.den * 1.0e-3, w.por * w.sat, w.vol,
+                     w.sat, w.temp, sd.f_t, sd.f_w,  s.somdec_f_t != nullptr,
+                     s.somdec_f_w != nullptr};
+  double nmin = 0.0;
+  for (int r = 0; r < n.sd.nrxn; ++r) nmin += sd_reaction(n.sd, e, r, dt, true);
+  sd_nemission(n.sd, e, nmin, dt, true);
+}
+
 // RReact (reaction.F90:3742-4055) for the plain kernel.  Returns the fail
 // reason (0 = converged); *nit receives the Newton iterations.
-template <int MAXN>
+template <int MAXN, bool SD>
 __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
-                            PlainCell<MAXN>& w, const double* guess, double dt,
-                            int* nit) {
+                            PlainCell<MAXN>& w, PlainSdState* sd, const double* guess,
+                            double dt, int* nit) {
   const int naq = n.naq, nimm = n.nimm, N = n.ncomp;
   double fixed[MAXN], initial[MAXN], upd[MAXN], prev[MAXN];
   double r0 = 0.0;
@@ -247,6 +292,7 @@ __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
         w.res[d] += kc * w.imm[ic];
         w.J[d * N + d] += kc;
       }
+      if constexpr (SD) plain_somdec(n, s, w, *sd, dt);
     }
     double nr = 0.0, mx = 0.0;
     for (int i = 0; i < N; ++i) {
@@ -295,13 +341,25 @@ __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
   return kOk;
 }
 
-template <int MAXN>
+template <int MAXN, bool SD>
 __global__ void __launch_bounds__(128)
-plain_step_kernel(DevNet n, DevState s, int64_t ncell, double dt) {
+plain_step_kernel(const DevNet* np, DevState s, int64_t ncell, double dt) {
+  const DevNet& n = *np;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const int naq = n.naq, nimm = n.nimm, N = n.ncomp;
   PlainCell<MAXN> w;
+  PlainSdState sd;
+  if constexpr (SD) {
+    for (int r = 0; r < n.sd.nrxn; ++r) {
+      sd.unc[r] = n.sd.upstream_nc[r];
+      sd.mcs[r] = n.sd.mineral_c_stoich[r];
+      sd.mns[r] = n.sd.mineral_n_stoich[r];
+      for (int j = 0; j < kSdDown; ++j) sd.dnc[r * kSdDown + j] = n.sd.downstream_nc[r * kSdDown + j];
+    }
+    sd.f_t = s.somdec_f_t ? s.somdec_f_t[c] : 1.0;
+    sd.f_w = s.somdec_f_w ? s.somdec_f_w[c] : 1.0;
+  }
   for (int i = 0; i < naq; ++i) {
     w.x[i] = s.pri_molal[i * s.ld + c];
     w.total[i] = s.total[i * s.ld + c];
@@ -354,7 +412,7 @@ plain_step_kernel(DevNet n, DevState s, int64_t ncell, double dt) {
         break;
       }
       int inner = 0;
-      const int why = plain_rreact<MAXN>(n, s, c, w, guess, tdt, &inner);
+      const int why = plain_rreact<MAXN, SD>(n, s, c, w, &sd, guess, tdt, &inner);
       nit_total += inner;
       if (why) {
         last_why = why;
@@ -409,7 +467,8 @@ plain_step_kernel(DevNet n, DevState s, int64_t ncell, double dt) {
 
 template <int MAXN>
 __global__ void __launch_bounds__(256)
-activity_kernel(DevNet n, DevState s, int64_t ncell) {
+activity_kernel(const DevNet* np, DevState s, int64_t ncell) {
+  const DevNet& n = *np;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   PlainCell<MAXN> w;
@@ -424,31 +483,34 @@ activity_kernel(DevNet n, DevState s, int64_t ncell) {
 
 int plain_max_ncomp() { return 64; }
 
-hipError_t launch_plain_step(const DevNet& n, const DevState& s, int64_t ncell,
+hipError_t launch_plain_step(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                              double dt, hipStream_t stream) {
   const int block = 128;
   const dim3 grid((unsigned)((ncell + block - 1) / block));
-  if (n.ncomp <= 4)
-    plain_step_kernel<4><<<grid, block, 0, stream>>>(n, s, ncell, dt);
-  else if (n.ncomp <= 8)
-    plain_step_kernel<8><<<grid, block, 0, stream>>>(n, s, ncell, dt);
-  else if (n.ncomp <= 16)
-    plain_step_kernel<16><<<grid, block, 0, stream>>>(n, s, ncell, dt);
-  else if (n.ncomp <= 32)
-    plain_step_kernel<32><<<grid, block, 0, stream>>>(n, s, ncell, dt);
-  else
-    plain_step_kernel<64><<<grid, block, 0, stream>>>(n, s, ncell, dt);
+  if (n.has_somdec) {
+    return hipErrorInvalidValue;  // refused by pfchem_step (SomDec runs in the team kernels)
+  } else if (n.ncomp <= 4) {
+    plain_step_kernel<4, false><<<grid, block, 0, stream>>>(dn, s, ncell, dt);
+  } else if (n.ncomp <= 8) {
+    plain_step_kernel<8, false><<<grid, block, 0, stream>>>(dn, s, ncell, dt);
+  } else if (n.ncomp <= 16) {
+    plain_step_kernel<16, false><<<grid, block, 0, stream>>>(dn, s, ncell, dt);
+  } else if (n.ncomp <= 32) {
+    plain_step_kernel<32, false><<<grid, block, 0, stream>>>(dn, s, ncell, dt);
+  } else {
+    plain_step_kernel<64, false><<<grid, block, 0, stream>>>(dn, s, ncell, dt);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_activity_lag(const DevNet& n, const DevState& s, int64_t ncell,
+hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                                hipStream_t stream) {
   const int block = 256;
   const dim3 grid((unsigned)((ncell + block - 1) / block));
   if (n.naq <= 16)
-    activity_kernel<16><<<grid, block, 0, stream>>>(n, s, ncell);
+    activity_kernel<16><<<grid, block, 0, stream>>>(dn, s, ncell);
   else
-    activity_kernel<64><<<grid, block, 0, stream>>>(n, s, ncell);
+    activity_kernel<64><<<grid, block, 0, stream>>>(dn, s, ncell);
   return hipGetLastError();
 }
 

@@ -44,10 +44,6 @@ hipError_t launch_team_8_4(const DevNet& n, const DevNet* dn, const DevState* ds
                           double dt, bool mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
-hipError_t launch_team_16_2(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
-                          unsigned long long* cursors, int num_cus, int nxcd,
-                          hipStream_t stream, bool has_status);
 hipError_t launch_team_32_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                           double dt, bool mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
@@ -56,13 +52,13 @@ hipError_t launch_team_32_1(const DevNet& n, const DevNet* dn, const DevState* d
 namespace {
 constexpr int kMaxMnrl = 16;
 // (lanes per cell, rows per lane) shapes compiled into the library
-const int kShapes[][2] = {{1, 4}, {4, 1}, {1, 8}, {2, 4}, {8, 1}, {2, 8}, {4, 4}, {8, 2}, {16, 1}, {8, 4}, {16, 2}, {32, 1}};
+const int kShapes[][2] = {{1, 4}, {4, 1}, {1, 8}, {2, 4}, {8, 1}, {2, 8}, {4, 4}, {8, 2}, {16, 1}, {8, 4}, {32, 1}};
 
 // per-team LDS bytes: the same layout as team_lds_layout in the kernel
-int team_lds_bytes(int NM, int nc, int nm, int jt_size) {
+int team_lds_bytes(int NM, int nc, int nm, int jt_size, int nsd) {
   int o = 8 * NM * 5;
   const int m = nm > 0 ? nm : 1;
-  o += 8 * m * 3 + 8 * (nc > 0 ? nc : 1) + 8 * (NM + 2) + jt_size * NM * (NM + 1);
+  o += 8 * m * 3 + 8 * (nc > 0 ? nc : 1) + 8 * (NM + 2) + jt_size * NM * (NM + 1) + 8 * nsd;
   return (o + 15) & ~15;
 }
 }  // namespace
@@ -70,14 +66,17 @@ int team_lds_bytes(int NM, int nc, int nm, int jt_size) {
 int team_max_ncomp() { return 32; }
 
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed) {
+  (void)mixed;  // both variants accumulate the Jacobian tile in FP64
   const int L = cfg.lanes_per_cell, R = cfg.rows_per_lane;
   bool known = false;
   for (const auto& sh : kShapes)
     if (sh[0] == L && sh[1] == R) known = true;
   if (!known) return 0;
   if (n.ncomp > L * R || n.nm > kMaxMnrl) return 0;
+  if (n.has_somdec && !team_has_somdec(L, R)) return 0;
   if (cfg.block_size % 64 || cfg.block_size > 256 || cfg.block_size < 64) return 0;
-  if ((size_t)(cfg.block_size / L) * team_lds_bytes(L * R, n.nc, n.nm, mixed ? 4 : 8) >
+  if ((size_t)(cfg.block_size / L) * team_lds_bytes(L * R, n.nc, n.nm, 8,
+                     n.has_somdec ? n.sd.nrxn * (3 + PFCHEM_SOMDEC_MAX_DOWN) : 0) >
       (size_t)cfg.lds_limit)
     return 0;
   return 1;
@@ -101,7 +100,6 @@ hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* d
   PFCHEM_SHAPE(8, 2)
   PFCHEM_SHAPE(16, 1)
   PFCHEM_SHAPE(8, 4)
-  PFCHEM_SHAPE(16, 2)
   PFCHEM_SHAPE(32, 1)
 #undef PFCHEM_SHAPE
   return hipErrorInvalidValue;

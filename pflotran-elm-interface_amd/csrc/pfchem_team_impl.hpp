@@ -39,6 +39,7 @@
 #pragma once
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
+#include "pfchem_somdec.hpp"
 
 #include <type_traits>
 
@@ -173,9 +174,10 @@ __device__ __forceinline__ int team_or(int v) {
 // LDS tile of one team (byte offsets); the same formula on host and device
 // ---------------------------------------------------------------------
 struct TeamLds {
-  int lnact, rs, tot, racc, xv, mvf, marea, mrate, lng, sol, J, bytes;
+  int lnact, rs, tot, racc, xv, mvf, marea, mrate, lng, sol, J, sd, bytes;
 };
-__host__ __device__ inline TeamLds team_lds_layout(int NM, int nc, int nm, int jt_size) {
+// nsd: doubles of per-cell SomDec stoichiometry (0 without the sandbox)
+__host__ __device__ inline TeamLds team_lds_layout(int NM, int nc, int nm, int jt_size, int nsd) {
   TeamLds t;
   int o = 0;
   t.lnact = o; o += 8 * NM;
@@ -190,6 +192,7 @@ __host__ __device__ inline TeamLds team_lds_layout(int NM, int nc, int nm, int j
   t.lng = o;   o += 8 * (nc > 0 ? nc : 1);
   t.sol = o;   o += 8 * (NM + 2);
   t.J = o;     o += jt_size * NM * (NM + 1);
+  t.sd = o;    o += 8 * nsd;
   o = (o + 15) & ~15;
   t.bytes = o;
   return t;
@@ -320,9 +323,9 @@ struct RxnRow {
   }
 
   // acc_i += nu_i v; J_ij += nu_i nu_j w * (row scale of i)
-  template <int NM, typename JT>
+  template <int NM>
   __device__ __forceinline__ void scatter(const int* ids, const double* st, double v, double w,
-                                          bool jac, double* acc, JT* Jt,
+                                          bool jac, double* acc, double* Jt,
                                           const double* rsv) const {
 #pragma unroll
     for (int a = 0; a < S; ++a) {
@@ -332,7 +335,7 @@ struct RxnRow {
           const double wi = nu[a] * w * rs[a];
 #pragma unroll
           for (int b = 0; b < S; ++b)
-            if (b < ncs) atomicAdd(&Jt[id[a] * (NM + 1) + id[b]], (JT)(wi * nu[b]));
+            if (b < ncs) atomicAdd(&Jt[id[a] * (NM + 1) + id[b]], wi * nu[b]);
         }
       }
     }
@@ -345,10 +348,39 @@ struct RxnRow {
         const double wi = nua * w * rsv[i];
         for (int b = 0; b < ncs; ++b)
           if (a >= S || b >= S)
-            atomicAdd(&Jt[i * (NM + 1) + gld(ids + 1 + b)], (JT)(wi * gld(st + b)));
+            atomicAdd(&Jt[i * (NM + 1) + gld(ids + 1 + b)], wi * gld(st + b));
       }
     }
   }
+};
+
+// SomDec accumulator of a team (see pfchem_somdec.hpp): species values from
+// the team's LDS tile, residual terms into the reaction residual, Jacobian
+// terms d res_row / d c_col into the tile in the scaled log form
+// (times c_col and the row scale).  Totals are free molality times density
+// (networks with the sandbox carry no complexes).
+struct TeamSdEnv {
+  const double* xv;
+  const double* lna;
+  const double* rsv;
+  double* racc;
+  double* Jt;
+  double* st;
+  int naq, nrxn, ld;
+  double dkl, theta, vol, sat, temp, f_t, f_w;
+  bool has_f_t, has_f_w;
+  __device__ double total(int i) const { return xv[i] * dkl; }
+  __device__ double imm(int i) const { return xv[naq + i]; }
+  __device__ double lnact(int i) const { return lna[i]; }
+  __device__ double dtot(int i, int j) const { return i == j ? dkl : 0.0; }
+  __device__ void R(int row, double v) { atomicAdd(&racc[row], v); }
+  __device__ void J(int row, int col, double v) {
+    atomicAdd(&Jt[row * ld + col], v * xv[col] * rsv[row]);
+  }
+  __device__ double& unc(int r) { return st[r]; }
+  __device__ double& mcs(int r) { return st[nrxn + r]; }
+  __device__ double& mns(int r) { return st[2 * nrxn + r]; }
+  __device__ double& dnc(int r, int j) { return st[3 * nrxn + r * kSdDown + j]; }
 };
 
 // Crout LU with implicit partial pivoting and back substitution
@@ -432,7 +464,9 @@ enum Outcome : int { kContinue = 0, kConverged = 1, kFailMaxIt = 2, kFailSingula
 #define PFCHEM_TEAM_WAVES_PER_EU 2
 #endif
 
-template <int L, int R, typename JT>
+// SD: the network carries the SomDec sandbox (instantiated for a few team
+// shapes only, so the other networks' kernels keep their register budget)
+template <int L, int R, typename JT, bool SD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PFCHEM_TEAM_WAVES_PER_EU)))
 team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                  unsigned long long* __restrict__ cursors, int nxcd, int rescue_only) {
@@ -448,7 +482,8 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   const int t = lane & (L - 1);          // lane within team
   const int team_in_wave = lane / L;
   const int wave_in_block = threadIdx.x >> 6;
-  const TeamLds lay = team_lds_layout(NM, n.nc, n.nm, (int)sizeof(JT));
+  const int nsd = SD ? n.sd.nrxn * (3 + kSdDown) : 0;
+  const TeamLds lay = team_lds_layout(NM, n.nc, n.nm, 8, nsd);
   char* tb = smem + (size_t)(wave_in_block * (64 / L) + team_in_wave) * lay.bytes;
   double* lnact = (double*)(tb + lay.lnact);
   double* rsv = (double*)(tb + lay.rs);
@@ -460,7 +495,10 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   double* rate = (double*)(tb + lay.mrate);
   double* lng = (double*)(tb + lay.lng);
   double* sol = (double*)(tb + lay.sol);
-  JT* Jt = (JT*)(tb + lay.J);
+  // Jacobian tile: FP64 for both variants (LDS FP64 atomic adds are cheap
+  // on gfx950, FP32 ones are not); JT is the precision of the LU
+  double* Jt = (double*)(tb + lay.J);
+  double* sdst = (double*)(tb + lay.sd);  // SomDec stoichiometry of the team's cell
 
   const int naq = n.naq, N = n.ncomp, nc = n.nc, nm = n.nm;
   const int64_t ld = s.ld;
@@ -475,6 +513,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   bool exhausted = false;
   double target = 0.0, cum = 0.0, tdt = 0.0, r0 = 0.0, ln_aw = 0.0;
   double por = 0.0, sat = 0.0, den = 0.0, vol = 0.0, temp = 25.0;
+  double sd_ft = 1.0, sd_fw = 1.0;  // ELM temperature / moisture scalars
   int nconst = 0, ncuts = 0, nsub = 0, nit_total = 0, nit = 0, status = 0;
   int attempts = 0, last_why = 0;
   // per-slot state: row i = r*L + t
@@ -567,6 +606,19 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           vf[m] = gld(s.mnrl_volfrac + (m * ld + c));
           area[m] = gld(s.mnrl_area + (m * ld + c));
           rate[m] = 0.0;
+        }
+        if constexpr (SD) {
+          // per-cell SomDec stoichiometry starts from the Setup values
+          const int nr = n.sd.nrxn;
+          for (int r = t; r < nr; r += L) {
+            sdst[r] = gld(n.sd.upstream_nc + r);
+            sdst[nr + r] = gld(n.sd.mineral_c_stoich + r);
+            sdst[2 * nr + r] = gld(n.sd.mineral_n_stoich + r);
+            for (int j = 0; j < kSdDown; ++j)
+              sdst[3 * nr + r * kSdDown + j] = gld(n.sd.downstream_nc + r * kSdDown + j);
+          }
+          sd_ft = s.somdec_f_t ? gld(s.somdec_f_t + c) : 1.0;
+          sd_fw = s.somdec_f_w ? gld(s.somdec_f_w + c) : 1.0;
         }
         nsub = nit_total = status = 0;
         ncuts = nconst = attempts = last_why = 0;
@@ -702,7 +754,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         racc[i] = 0.0;
       }
 #pragma unroll
-      for (int j = 0; j < NM; ++j) Jt[i * (NM + 1) + j] = (JT)0;
+      for (int j = 0; j < NM; ++j) Jt[i * (NM + 1) + j] = 0.0;
     }
     team_sync();
     // phases B and C: complexes (RTotalAqueous) and kinetic minerals
@@ -775,7 +827,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int q = 0; q < KC; ++q) {
         const int k = k0 + q * L + t;
         if (k < nc)
-          crow[q].template scatter<NM, JT>(n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, mk[q],
+          crow[q].template scatter<NM>(n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, mk[q],
                                            mk[q] * jc, wet, tot, Jt, rsv);
       }
     }
@@ -783,7 +835,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     for (int q = 0; q < KM; ++q) {
       const int m = q * L + t;
       if (m < nm && mon[q])
-        mrow[q].template scatter<NM, JT>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, mv[q],
+        mrow[q].template scatter<NM>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, mv[q],
                                          mg[q], true, racc, Jt, rsv);
     }
     // minerals beyond the register slots (more than KM*L of them)
@@ -801,12 +853,23 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         if (mr.active) {
           rt = mr.Im;
           if (wet)
-            row.template scatter<NM, JT>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms,
+            row.template scatter<NM>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms,
                                          mr.Im * vol, mr.dIm_dQK * vol * mr.QK * dkl, true, racc,
                                          Jt, rsv);
         }
       }
       rate[m] = rt;
+    }
+    if (SD && wet) {
+      // SomDec sandbox (SomDecReact, reaction_sandbox_somdec.F90:1504-1910):
+      // reactions r = t, t+L, ...; the N2O emission needs the net N
+      // mineralization of all of them
+      TeamSdEnv e{xv, lnact, rsv, racc, Jt, sdst, naq, n.sd.nrxn, NM + 1, dkl, por * sat, vol,
+                  sat, temp, sd_ft, sd_fw, s.somdec_f_t != nullptr, s.somdec_f_w != nullptr};
+      double nmin = 0.0;
+      for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, tdt, true);
+      nmin = team_sum<L>(nmin);
+      if (t == 0) sd_nemission(n.sd, e, nmin, tdt, true);
     }
     team_sync();
 
@@ -820,7 +883,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int r = 0; r < R; ++r) {
         const int i = r * L + t;
 #pragma unroll
-        for (int j = 0; j < NM; ++j) a[r][j] = i < N ? Jt[i * (NM + 1) + j] : (JT)0;
+        for (int j = 0; j < NM; ++j) a[r][j] = i < N ? (JT)Jt[i * (NM + 1) + j] : (JT)0;
         if (i >= N) continue;
         if (wet) {
           double d = 1.0;
@@ -957,14 +1020,14 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
             const int i = r * L + t;
             if (i < N) {
 #pragma unroll
-              for (int j = 0; j < NM; ++j) Jt[i * (NM + 1) + j] = a[r][j];
+              for (int j = 0; j < NM; ++j) Jt[i * (NM + 1) + j] = (double)a[r][j];
               sol[i] = res[r] * rs[r];
             }
           }
           team_sync();
           int serr = 0;
           // racc / tot were consumed by the residual: work arrays of the LU
-          if (t == 0) serr = serial_lu_solve<JT>(Jt, NM + 1, N, sol, racc, (int*)tot);
+          if (t == 0) serr = serial_lu_solve<double>(Jt, NM + 1, N, sol, racc, (int*)tot);
           serr = bcast<L, 0>(serr);
           team_sync();
 #pragma unroll
@@ -1110,11 +1173,11 @@ __global__ void zero_cursors(unsigned long long* c, int n) {
   if ((int)threadIdx.x < n) c[16 * threadIdx.x] = 0ull;
 }
 
-template <int L, int R, typename JT>
+template <int L, int R, typename JT, bool SD>
 hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                      double dt, const LaunchCfg& cfg, unsigned long long* cursors,
                      int num_cus, int nxcd, hipStream_t stream, int rescue_only) {
-  const TeamLds lay = team_lds_layout(L * R, n.nc, n.nm, (int)sizeof(JT));
+  const TeamLds lay = team_lds_layout(L * R, n.nc, n.nm, 8, SD ? n.sd.nrxn * (3 + kSdDown) : 0);
   const int block = cfg.block_size;
   const size_t lds = (size_t)(block / L) * lay.bytes;
   int64_t blocks = (int64_t)num_cus * cfg.blocks_per_cu;
@@ -1125,12 +1188,12 @@ hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int6
   if (blocks < 1) blocks = 1;
   if (lds > 65536) {
     const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&team_step_kernel<L, R, JT>),
+        reinterpret_cast<const void*>(&team_step_kernel<L, R, JT, SD>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   zero_cursors<<<1, 64, 0, stream>>>(cursors, nxcd);
-  team_step_kernel<L, R, JT><<<dim3((unsigned)blocks), dim3(block), lds, stream>>>(
+  team_step_kernel<L, R, JT, SD><<<dim3((unsigned)blocks), dim3(block), lds, stream>>>(
       dn, ds, ncell, dt, cursors, nxcd, rescue_only);
   return hipGetLastError();
 }
@@ -1140,14 +1203,31 @@ hipError_t launch_pair(const DevNet& n, const DevNet* dn, const DevState* ds, in
                        double dt, bool mixed, const LaunchCfg& cfg,
                        unsigned long long* cursors, int num_cus, int nxcd,
                        hipStream_t stream, bool has_status) {
-  if (!mixed)
-    return launch_LR<L, R, double>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd, stream, 0);
-  hipError_t e =
-      launch_LR<L, R, float>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd, stream, 0);
-  if (e != hipSuccess || !has_status) return e;
-  LaunchCfg c64 = cfg;
-  while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
-  return launch_LR<L, R, double>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd, stream, 1);
+  if (!n.has_somdec) {
+    if (!mixed)
+      return launch_LR<L, R, double, false>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd,
+                                            stream, 0);
+    hipError_t e = launch_LR<L, R, float, false>(n, dn, ds, ncell, dt, cfg, cursors, num_cus,
+                                                 nxcd, stream, 0);
+    if (e != hipSuccess || !has_status) return e;
+    LaunchCfg c64 = cfg;
+    while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
+    return launch_LR<L, R, double, false>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd,
+                                          stream, 1);
+  }
+  if constexpr (team_has_somdec(L, R)) {
+    if (!mixed)
+      return launch_LR<L, R, double, true>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd,
+                                           stream, 0);
+    hipError_t e = launch_LR<L, R, float, true>(n, dn, ds, ncell, dt, cfg, cursors, num_cus,
+                                                nxcd, stream, 0);
+    if (e != hipSuccess || !has_status) return e;
+    LaunchCfg c64 = cfg;
+    while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
+    return launch_LR<L, R, double, true>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd,
+                                         stream, 1);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace

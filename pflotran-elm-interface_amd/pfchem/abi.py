@@ -199,4 +199,4 @@ class Launch(C.Structure):
 
 # (lanes per cell, rows per lane) team shapes compiled into libpfchem.so
 TEAM_SHAPES = [(1, 4), (4, 1), (1, 8), (2, 4), (8, 1), (2, 8), (4, 4), (8, 2), (16, 1),
-               (8, 4), (16, 2), (32, 1)]
+               (8, 4), (32, 1)]

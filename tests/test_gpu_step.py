@@ -85,7 +85,7 @@ def _run_shape(wl, variant, lanes, rows, block):
 
 
 @pytest.mark.parametrize("lanes,rows,block", [(16, 1, 64), (16, 1, 256), (32, 1, 128), (8, 2, 128),
-                                             (4, 4, 128), (2, 8, 64), (8, 4, 64), (16, 2, 128)])
+                                             (4, 4, 128), (2, 8, 64), (8, 4, 64), (32, 1, 64)])
 def test_launch_shapes_agree(randnet, lanes, rows, block):
     """The autotuner's choices (lanes per cell, rows per lane, workgroup size)
     must not change the answer beyond the parity bar."""

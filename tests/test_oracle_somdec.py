@@ -15,6 +15,7 @@ import pytest
 
 import golden_util as GU
 import oracle_lib as O
+import somdec_util as SU
 from pfchem import abi, workloads
 from pfchem.network import SolverOptions
 from pfchem.somdec import CN_RATIO_MASS_TO_MOL
@@ -73,27 +74,12 @@ def test_clm_cn3_gold_catches_a_wrong_stoichiometry():
     assert abs(got - rows[1, 3]) / rows[1, 3] > 1e-4
 
 
-def _carbon_nitrogen(wl, st):
-    """C and N (mol per cell) in pools, CO2, NH4+, NO3-, N2O"""
-    a = st.arrays
-    n = st.ncell
-    net = wl.net
-    sd = net.somdec
-    psv = a["porosity"][0, :n] * a["saturation"][0, :n] * 1000.0 * a["volume"][0, :n]
-    imm = a["immobile"][:, :n] * a["volume"][0, :n]
-    tot = a["total"][:, :n] * psv
-    nc_som = sd.arrays["upstream_nc"][3:7]     # SOM1-4 fixed N:C (reactions 4-7)
-    C = imm[0:4].sum(0) + imm[4:7].sum(0) + tot[0]
-    N = (imm[0:4] * nc_som[:, None]).sum(0) + imm[7:10].sum(0) + tot[1] + tot[2] + 2 * tot[3]
-    return C, N
-
-
 def test_somdec_step_conserves_carbon_and_nitrogen():
     wl = workloads.somdec_workload(4000)
     st = wl.state.copy()
     assert O.step_batch(wl.net, wl.opt, st, wl.dt, 0) == 0
-    C0, N0 = _carbon_nitrogen(wl, wl.state)
-    C1, N1 = _carbon_nitrogen(wl, st)
+    C0, N0 = SU.carbon_nitrogen(wl.net, wl.state.arrays, st.ncell)
+    C1, N1 = SU.carbon_nitrogen(wl.net, st.arrays, st.ncell)
     np.testing.assert_allclose(C1, C0, rtol=1e-11)
     np.testing.assert_allclose(N1, N0, rtol=1e-9)
     # and something happened
