@@ -360,12 +360,15 @@ struct RxnRow {
 // (times c_col and the row scale).  Totals are free molality times density
 // (networks with the sandbox carry no complexes).
 struct TeamSdEnv {
-  const double* xv;
-  const double* lna;
-  const double* rsv;
-  double* racc;
-  double* Jt;
-  double* st;
+  // LDS-qualified pointers: sd_reaction is too large to inline, and through
+  // generic pointers its loads and FP64 atomic adds would take the FLAT path
+  typedef __attribute__((address_space(3))) double lds_t;
+  const lds_t* xv;
+  const lds_t* lna;
+  const lds_t* rsv;
+  lds_t* racc;
+  lds_t* Jt;
+  lds_t* st;
   int naq, nrxn, ld;
   double dkl, theta, vol, sat, temp, f_t, f_w;
   bool has_f_t, has_f_w;
@@ -373,14 +376,18 @@ struct TeamSdEnv {
   __device__ double imm(int i) const { return xv[naq + i]; }
   __device__ double lnact(int i) const { return lna[i]; }
   __device__ double dtot(int i, int j) const { return i == j ? dkl : 0.0; }
-  __device__ void R(int row, double v) { atomicAdd(&racc[row], v); }
-  __device__ void J(int row, int col, double v) {
-    atomicAdd(&Jt[row * ld + col], v * xv[col] * rsv[row]);
+  __device__ void R(int row, double v) {
+    __hip_atomic_fetch_add(&racc[row], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
-  __device__ double& unc(int r) { return st[r]; }
-  __device__ double& mcs(int r) { return st[nrxn + r]; }
-  __device__ double& mns(int r) { return st[2 * nrxn + r]; }
-  __device__ double& dnc(int r, int j) { return st[3 * nrxn + r * kSdDown + j]; }
+  __device__ void J(int row, int col, double v) {
+    __hip_atomic_fetch_add(&Jt[row * ld + col], v * xv[col] * rsv[row], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ lds_t& unc(int r) { return st[r]; }
+  __device__ lds_t& mcs(int r) { return st[nrxn + r]; }
+  __device__ lds_t& mns(int r) { return st[2 * nrxn + r]; }
+  __device__ lds_t& dnc(int r, int j) { return st[3 * nrxn + r * kSdDown + j]; }
+  static __device__ lds_t* lds(double* p) { return (lds_t*)p; }
 };
 
 // Crout LU with implicit partial pivoting and back substitution
@@ -864,8 +871,10 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       // SomDec sandbox (SomDecReact, reaction_sandbox_somdec.F90:1504-1910):
       // reactions r = t, t+L, ...; the N2O emission needs the net N
       // mineralization of all of them
-      TeamSdEnv e{xv, lnact, rsv, racc, Jt, sdst, naq, n.sd.nrxn, NM + 1, dkl, por * sat, vol,
-                  sat, temp, sd_ft, sd_fw, s.somdec_f_t != nullptr, s.somdec_f_w != nullptr};
+      TeamSdEnv e{TeamSdEnv::lds(xv), TeamSdEnv::lds(lnact), TeamSdEnv::lds(rsv),
+                  TeamSdEnv::lds(racc), TeamSdEnv::lds(Jt), TeamSdEnv::lds(sdst), naq,
+                  n.sd.nrxn, NM + 1, dkl, por * sat, vol, sat, temp, sd_ft, sd_fw,
+                  s.somdec_f_t != nullptr, s.somdec_f_w != nullptr};
       double nmin = 0.0;
       for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, tdt, true);
       nmin = team_sum<L>(nmin);

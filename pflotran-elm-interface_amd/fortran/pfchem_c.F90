@@ -1,0 +1,264 @@
+! pfchem_c.F90 -- ISO_C_BINDING view of include/pfchem.h for Fortran hosts.
+!
+! bind(C) derived types mirror the C structs field by field (array members are
+! type(c_ptr) obtained with c_loc; NULL means "not associated"), and the
+! interfaces bind the C entry points by name.  Species ids in these structs
+! are 0-based, as in C; pfchem_reaction_module converts from the reference's
+! 1-based ids.
+module pfchem_c
+  use, intrinsic :: iso_c_binding
+  implicit none
+
+  ! return codes
+  integer(c_int), parameter :: PFCHEM_OK = 0
+  integer(c_int), parameter :: PFCHEM_ERR_INVALID = 1
+  integer(c_int), parameter :: PFCHEM_ERR_HIP = 2
+  integer(c_int), parameter :: PFCHEM_ERR_UNSUPPORTED = 3
+  integer(c_int), parameter :: PFCHEM_ERR_NOMEM = 4
+  integer(c_int), parameter :: PFCHEM_ERR_CELLS = 5
+
+  ! per-cell status
+  integer(c_int), parameter :: PFCHEM_CELL_OK = 0
+  integer(c_int), parameter :: PFCHEM_CELL_MAX_CUTS_ITERATIONS = 1
+  integer(c_int), parameter :: PFCHEM_CELL_MAX_CUTS_SINGULAR = 2
+  integer(c_int), parameter :: PFCHEM_CELL_MAX_CUTS_ACTIVITY = 3
+
+  ! activity coefficient options (the reaction_aux.F90:27-31 values)
+  integer(c_int), parameter :: PFCHEM_ACT_COEF_FREQUENCY_OFF = 0
+  integer(c_int), parameter :: PFCHEM_ACT_COEF_FREQUENCY_TIMESTEP = 1
+  integer(c_int), parameter :: PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER = 2
+  integer(c_int), parameter :: PFCHEM_ACT_COEF_ALGORITHM_LAG = 3
+  integer(c_int), parameter :: PFCHEM_ACT_COEF_ALGORITHM_NEWTON = 4
+
+  ! kernel variants
+  integer(c_int), parameter :: PFCHEM_KERNEL_PLAIN_FP64 = 0
+  integer(c_int), parameter :: PFCHEM_KERNEL_BATCHED_FP64 = 1
+  integer(c_int), parameter :: PFCHEM_KERNEL_BATCHED_MIXED = 2
+
+  integer, parameter :: PFCHEM_SOMDEC_MAX_DOWN = 4
+  integer, parameter :: PFCHEM_SOMDEC_MAX_MONOD = 4
+  integer, parameter :: PFCHEM_SOMDEC_MAX_INHIB = 4
+
+  type, bind(C) :: pfchem_network
+    integer(c_int32_t) :: naqcomp
+    integer(c_int32_t) :: nimmobile
+    integer(c_int32_t) :: h2o_aq_id
+    integer(c_int32_t) :: pad0
+    type(c_ptr) :: primary_spec_Z
+    type(c_ptr) :: primary_spec_a0
+    integer(c_int32_t) :: neqcplx
+    integer(c_int32_t) :: eqcplx_maxsp
+    type(c_ptr) :: eqcplxspecid
+    type(c_ptr) :: eqcplxstoich
+    type(c_ptr) :: eqcplxh2ostoich
+    type(c_ptr) :: eqcplx_logK
+    type(c_ptr) :: eqcplx_logKcoef
+    type(c_ptr) :: eqcplx_Z
+    type(c_ptr) :: eqcplx_a0
+    integer(c_int32_t) :: nkinmnrl
+    integer(c_int32_t) :: kinmnrl_maxsp
+    type(c_ptr) :: kinmnrlspecid
+    type(c_ptr) :: kinmnrlstoich
+    type(c_ptr) :: kinmnrlh2ostoich
+    type(c_ptr) :: kinmnrl_logK
+    type(c_ptr) :: kinmnrl_logKcoef
+    type(c_ptr) :: kinmnrl_rate_constant
+    type(c_ptr) :: kinmnrl_activation_energy
+    type(c_ptr) :: kinmnrl_molar_vol
+    type(c_ptr) :: kinmnrl_Temkin_const
+    type(c_ptr) :: kinmnrl_min_scale_factor
+    type(c_ptr) :: kinmnrl_affinity_power
+    type(c_ptr) :: kinmnrl_affinity_threshold
+    type(c_ptr) :: kinmnrl_rate_limiter
+    type(c_ptr) :: kinmnrl_irreversible
+    integer(c_int32_t) :: ndecay_rxn
+    integer(c_int32_t) :: pad1
+    type(c_ptr) :: decayspecid
+    type(c_ptr) :: decay_rate_constant
+    real(c_double) :: debyeA
+    real(c_double) :: debyeB
+    real(c_double) :: debyeBdot
+    type(c_ptr) :: somdec
+  end type pfchem_network
+
+  type, bind(C) :: pfchem_options
+    integer(c_int32_t) :: use_full_geochemistry
+    integer(c_int32_t) :: use_log_formulation
+    integer(c_int32_t) :: act_coef_update_frequency
+    integer(c_int32_t) :: act_coef_update_algorithm
+    integer(c_int32_t) :: use_activity_h2o
+    integer(c_int32_t) :: use_total_as_guess
+    integer(c_int32_t) :: isothermal
+    integer(c_int32_t) :: maximum_reaction_iterations
+    integer(c_int32_t) :: maximum_reaction_cuts
+    integer(c_int32_t) :: pad0
+    real(c_double) :: max_dlnC_rreact
+    real(c_double) :: max_residual_tolerance
+    real(c_double) :: max_rel_residual_tolerance
+    real(c_double) :: max_relative_change_tolerance
+  end type pfchem_options
+
+  type, bind(C) :: pfchem_somdec
+    integer(c_int32_t) :: nrxn
+    integer(c_int32_t) :: species_id_co2
+    integer(c_int32_t) :: species_id_nh4
+    integer(c_int32_t) :: species_id_no3
+    integer(c_int32_t) :: species_id_n2o
+    integer(c_int32_t) :: species_id_proton
+    integer(c_int32_t) :: species_id_hr
+    integer(c_int32_t) :: species_id_nmin
+    integer(c_int32_t) :: species_id_nimm
+    integer(c_int32_t) :: species_id_nimp
+    integer(c_int32_t) :: species_id_ngasmin
+    integer(c_int32_t) :: pad0
+    real(c_double) :: x0eps
+    real(c_double) :: n2o_frac_mineralization
+    real(c_double) :: inhibition_nh4_no3
+    type(c_ptr) :: rate_constant
+    type(c_ptr) :: rate_decomposition
+    type(c_ptr) :: rate_ad_factor
+    type(c_ptr) :: upstream_c_id
+    type(c_ptr) :: upstream_n_id
+    type(c_ptr) :: upstream_hr_id
+    type(c_ptr) :: upstream_nmin_id
+    type(c_ptr) :: upstream_nimp_id
+    type(c_ptr) :: upstream_nimm_id
+    type(c_ptr) :: upstream_nc
+    type(c_ptr) :: upstream_is_varycn
+    type(c_ptr) :: n_downstream_pools
+    type(c_ptr) :: downstream_c_id
+    type(c_ptr) :: downstream_n_id
+    type(c_ptr) :: downstream_stoich
+    type(c_ptr) :: downstream_nc
+    type(c_ptr) :: downstream_is_varycn
+    type(c_ptr) :: mineral_c_stoich
+    type(c_ptr) :: mineral_n_stoich
+    type(c_ptr) :: temperature_response_function
+    type(c_ptr) :: Q10
+    type(c_ptr) :: Ea
+    type(c_ptr) :: moisture_response_function
+    type(c_ptr) :: Ox_response_function
+    type(c_ptr) :: Ox_half_saturation
+    type(c_ptr) :: n_monod
+    type(c_ptr) :: monod_specid
+    type(c_ptr) :: monod_half_saturation
+    type(c_ptr) :: monod_threshold
+    type(c_ptr) :: monod_pool_normalized
+    type(c_ptr) :: n_inhibition
+    type(c_ptr) :: inhibition_specid
+    type(c_ptr) :: inhibition_type
+    type(c_ptr) :: inhibition_constant
+    type(c_ptr) :: inhibition_constant2
+  end type pfchem_somdec
+
+  type, bind(C) :: pfchem_state
+    integer(c_int64_t) :: ld
+    type(c_ptr) :: pri_molal
+    type(c_ptr) :: total
+    type(c_ptr) :: immobile
+    type(c_ptr) :: sec_molal
+    type(c_ptr) :: pri_act_coef
+    type(c_ptr) :: sec_act_coef
+    type(c_ptr) :: ln_act_h2o
+    type(c_ptr) :: mnrl_volfrac
+    type(c_ptr) :: mnrl_area
+    type(c_ptr) :: mnrl_rate
+    type(c_ptr) :: porosity
+    type(c_ptr) :: saturation
+    type(c_ptr) :: den_kg
+    type(c_ptr) :: volume
+    type(c_ptr) :: temperature
+    type(c_ptr) :: somdec_f_t
+    type(c_ptr) :: somdec_f_w
+    type(c_ptr) :: num_iterations
+    type(c_ptr) :: num_sub_steps
+    type(c_ptr) :: status
+  end type pfchem_state
+
+  type, bind(C) :: pfchem_launch
+    integer(c_int32_t) :: lanes_per_cell
+    integer(c_int32_t) :: block_size
+    integer(c_int32_t) :: blocks_per_cu
+    integer(c_int32_t) :: rows_per_lane
+  end type pfchem_launch
+
+  interface
+    integer(c_int) function pfchem_create(net, opt, device, ctx) bind(C, name="pfchem_create")
+      import :: c_int, c_ptr, pfchem_network, pfchem_options
+      type(pfchem_network), intent(in) :: net
+      type(pfchem_options), intent(in) :: opt
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function pfchem_create
+
+    integer(c_int) function pfchem_destroy(ctx) bind(C, name="pfchem_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function pfchem_destroy
+
+    type(c_ptr) function pfchem_strerror(code) bind(C, name="pfchem_strerror")
+      import :: c_int, c_ptr
+      integer(c_int), value :: code
+    end function pfchem_strerror
+
+    type(c_ptr) function pfchem_last_error(ctx) bind(C, name="pfchem_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function pfchem_last_error
+
+    integer(c_int) function pfchem_step(ctx, ncell, state, dt, variant, stream) &
+        bind(C, name="pfchem_step")
+      import :: c_int, c_int64_t, c_double, c_ptr, pfchem_state
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ncell
+      type(pfchem_state), intent(in) :: state
+      real(c_double), value :: dt
+      integer(c_int), value :: variant
+      type(c_ptr), value :: stream
+    end function pfchem_step
+
+    integer(c_int) function pfchem_step_host(ctx, ncell, state, dt, variant) &
+        bind(C, name="pfchem_step_host")
+      import :: c_int, c_int64_t, c_double, c_ptr, pfchem_state
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ncell
+      type(pfchem_state), intent(in) :: state
+      real(c_double), value :: dt
+      integer(c_int), value :: variant
+    end function pfchem_step_host
+
+    integer(c_int) function pfchem_set_launch(ctx, variant, cfg) bind(C, name="pfchem_set_launch")
+      import :: c_int, c_ptr, pfchem_launch
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: variant
+      type(pfchem_launch), intent(in) :: cfg
+    end function pfchem_set_launch
+
+    integer(c_size_t) function pfchem_c_strlen(s) bind(C, name="strlen")
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function pfchem_c_strlen
+  end interface
+
+contains
+
+  ! Fortran string of a NUL-terminated C string (pfchem_strerror /
+  ! pfchem_last_error)
+  function pfchem_string(p) result(s)
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: s
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    if (.not. c_associated(p)) then
+      s = ''
+      return
+    end if
+    n = int(pfchem_c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: s)
+    do i = 1, n
+      s(i:i) = chars(i)
+    end do
+  end function pfchem_string
+
+end module pfchem_c
