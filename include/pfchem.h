@@ -157,7 +157,10 @@ typedef struct pfchem_options {
   int32_t isothermal;                    /* option%use_isothermal           */
   int32_t maximum_reaction_iterations;   /* default 20                      */
   int32_t maximum_reaction_cuts;         /* default 10                      */
-  int32_t pad0;
+  int32_t variable_ncomp;                /* 1: pfchem_state.ncomp_active gives
+                                            each cell's active primaries (see
+                                            there); 0: every cell solves the
+                                            whole network                   */
   double max_dlnC_rreact;                /* default 5                       */
   double max_residual_tolerance;         /* default 1e-12                   */
   double max_rel_residual_tolerance;     /* default 1e-8                    */
@@ -260,6 +263,15 @@ typedef struct pfchem_state {
   int32_t* num_iterations;    /* [1] out: Newton iterations (RStep)        */
   int32_t* num_sub_steps;     /* [1] out                                   */
   int32_t* status;            /* [1] out: PFCHEM_CELL_*                    */
+  const int32_t* ncomp_active;/* [1] with options.variable_ncomp: the cell
+                                 solves the network restricted to primaries
+                                 0..ncomp_active-1 (clamped to [1, naqcomp]):
+                                 the complexes and kinetic minerals all of
+                                 whose species are among them; the other
+                                 entries of the cell are left untouched.
+                                 Networks with immobile species or the
+                                 SomDec sandbox are not supported.  NULL
+                                 otherwise.                                 */
 } pfchem_state;
 
 /* ------------------------------------------------------------------ */
@@ -311,9 +323,9 @@ int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
    team of lanes_per_cell lanes owns one cell (64 / lanes_per_cell cells per
    wavefront) and each lane owns rows_per_lane rows of the cell's system
    (lanes_per_cell * rows_per_lane >= ncomp).  Shapes compiled in:
-   1x4 4x1 1x8 2x4 8x1 2x8 4x4 8x2 16x1 8x4 32x1. */
+   4x1 2x4 8x1 2x8 4x4 8x2 16x1 8x4 32x1 64x1. */
 typedef struct pfchem_launch {
-  int32_t lanes_per_cell;   /* 1, 2, 4, 8, 16 or 32                       */
+  int32_t lanes_per_cell;   /* 2, 4, 8, 16, 32 or 64                      */
   int32_t block_size;       /* threads per workgroup (64, 128 or 256)     */
   int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CUs      */
   int32_t rows_per_lane;    /* 1, 2, 4 or 8                               */

@@ -203,7 +203,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     delete ctx;
     return PFCHEM_ERR_INVALID;
   }
-  if (naq + nimm > 32 || net->nkinmnrl > 16 || net->neqcplx > 4096) {
+  if (naq + nimm > 64 || net->nkinmnrl > 16 || net->neqcplx > 4096) {
     delete ctx;
     return PFCHEM_ERR_UNSUPPORTED;
   }
@@ -236,6 +236,11 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
       delete ctx;
       return rc;
     }
+  }
+  // per-cell active primaries: aqueous networks without the sandbox
+  if (opt->variable_ncomp && (nimm > 0 || net->somdec)) {
+    delete ctx;
+    return PFCHEM_ERR_UNSUPPORTED;
   }
   if (opt->max_dlnC_rreact <= 0.0 || opt->maximum_reaction_iterations < 1 ||
       opt->maximum_reaction_cuts < 0) {
@@ -280,6 +285,22 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   size_t omt = addd(net->kinmnrl_affinity_threshold, nm), oml = addd(net->kinmnrl_rate_limiter, nm);
   size_t omI = addi(net->kinmnrl_irreversible, nm);
   size_t odi = addi(net->decayspecid, net->ndecay_rxn), odk = addd(net->decay_rate_constant, net->ndecay_rxn);
+  // largest species id per complex / mineral (per-cell active primaries)
+  size_t ocmx = kNone, ommx = kNone;
+  if (opt->variable_ncomp) {
+    auto maxids = [](const int32_t* ids, int nr, int ms) {
+      std::vector<int32_t> v(nr > 0 ? nr : 1, -1);
+      for (int r = 0; r < nr; ++r) {
+        const int32_t* row = ids + (size_t)r * (ms + 1);
+        for (int a = 1; a <= row[0]; ++a) v[r] = row[a] > v[r] ? row[a] : v[r];
+      }
+      return v;
+    };
+    const std::vector<int32_t> cm = maxids(net->eqcplxspecid, nc, net->eqcplx_maxsp);
+    const std::vector<int32_t> mm = maxids(net->kinmnrlspecid, nm, net->kinmnrl_maxsp);
+    ocmx = P.add(cm.data(), cm.size() * sizeof(int32_t));
+    ommx = P.add(mm.data(), mm.size() * sizeof(int32_t));
+  }
   // SomDec per-reaction arrays
   const pfchem_somdec* sdh = net->somdec;
   const int nr = sdh ? sdh->nrxn : 0;
@@ -403,6 +424,9 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   d.mnrl_irrev = pi(omI);
   d.decay_id = pi(odi);
   d.decay_k = pd(odk);
+  d.variable = opt->variable_ncomp != 0;
+  d.cplx_maxid = pi(ocmx);
+  d.mnrl_maxid = pi(ommx);
   d.has_somdec = sdh != nullptr;
   if (sdh) {
     DevSomdec& sd = d.sd;
@@ -463,7 +487,9 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   // default team shape (lanes per cell x rows per lane) by system size;
   // the autotuner replaces it with a measured choice
   int L = 32, R = 1;
-  if (d.has_somdec) {
+  if (d.ncomp > 32) {
+    L = 64;  // one cell per wavefront
+  } else if (d.has_somdec) {
     if (d.ncomp <= 16) L = 16;  // SomDec kernels exist for 16x1, 8x2, 32x1
   } else if (d.ncomp <= 4) {
     L = 4;
@@ -523,6 +549,8 @@ static int to_dev_state(pfchem_ctx* ctx, const pfchem_state* st, int64_t ncell, 
     return fail(ctx, PFCHEM_ERR_INVALID, "missing sec_molal / sec_act_coef");
   if (n.nm > 0 && (!st->mnrl_volfrac || !st->mnrl_area))
     return fail(ctx, PFCHEM_ERR_INVALID, "missing mineral arrays");
+  if (n.variable && !st->ncomp_active)
+    return fail(ctx, PFCHEM_ERR_INVALID, "variable_ncomp needs ncomp_active");
   d->ld = st->ld;
   d->pri_molal = st->pri_molal;
   d->total = st->total;
@@ -541,6 +569,7 @@ static int to_dev_state(pfchem_ctx* ctx, const pfchem_state* st, int64_t ncell, 
   d->temperature = st->temperature;
   d->somdec_f_t = st->somdec_f_t;
   d->somdec_f_w = st->somdec_f_w;
+  d->ncomp_active = n.variable ? st->ncomp_active : nullptr;
   d->num_iterations = st->num_iterations;
   d->num_sub_steps = st->num_sub_steps;
   d->status = st->status;
@@ -650,7 +679,8 @@ int pfchem_step_host(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* h, doub
       {h->den_kg, 1, 8, false, 0},             {h->volume, 1, 8, false, 0},
       {h->temperature, 1, 8, false, 0},        {h->somdec_f_t, 1, 8, false, 0},
       {h->somdec_f_w, 1, 8, false, 0},         {h->num_iterations, 1, 4, true, 0},
-      {h->num_sub_steps, 1, 4, true, 0},       {h->status, 1, 4, true, 0}};
+      {h->num_sub_steps, 1, 4, true, 0},       {h->status, 1, 4, true, 0},
+      {h->ncomp_active, 1, 4, false, 0}};
   const int nf = sizeof(f) / sizeof(f[0]);
   size_t bytes = 0;
   for (int i = 0; i < nf; ++i) {
@@ -676,7 +706,7 @@ int pfchem_step_host(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* h, doub
                  (void**)&d.porosity, (void**)&d.saturation, (void**)&d.den_kg,
                  (void**)&d.volume, (void**)&d.temperature, (void**)&d.somdec_f_t,
                  (void**)&d.somdec_f_w, (void**)&d.num_iterations, (void**)&d.num_sub_steps,
-                 (void**)&d.status};
+                 (void**)&d.status, (void**)&d.ncomp_active};
   for (int i = 0; i < nf; ++i) {
     if (!f[i].src || f[i].rows <= 0) {
       *dp[i] = nullptr;

@@ -110,6 +110,12 @@ struct DevNet {
   // SomDec sandbox
   int has_somdec;
   DevSomdec sd;
+  // per-cell active primaries (options.variable_ncomp): the largest species
+  // id of every complex / kinetic mineral; a cell with nact active primaries
+  // keeps the reactions whose largest id is below nact
+  int variable;
+  const int* cplx_maxid;
+  const int* mnrl_maxid;
 };
 
 // SoA per-cell state, device pointers (pfchem_state)
@@ -132,6 +138,7 @@ struct DevState {
   const double* temperature;
   const double* somdec_f_t;
   const double* somdec_f_w;
+  const int* ncomp_active;
   int32_t* num_iterations;
   int32_t* num_sub_steps;
   int32_t* status;

@@ -26,7 +26,20 @@ struct PlainCell {
   double vf[kMaxMnrl], rate[kMaxMnrl], logKm[kMaxMnrl];
   double ln_act_h2o;
   double por, sat, den, vol, temp;
+  int naq, N;  // the cell's aqueous primaries and unknowns (per-cell active
+               // counts with options.variable_ncomp)
 };
+
+// complex k / mineral m takes part in the cell's system: with per-cell
+// active primaries, only those all of whose species are active
+template <int MAXN>
+__device__ __forceinline__ bool plain_cplx_on(const DevNet& n, const PlainCell<MAXN>& w, int k) {
+  return !n.variable || n.cplx_maxid[k] < w.naq;
+}
+template <int MAXN>
+__device__ __forceinline__ bool plain_mnrl_on(const DevNet& n, const PlainCell<MAXN>& w, int m) {
+  return !n.variable || n.mnrl_maxid[m] < w.naq;
+}
 
 // RTotalAqueous (reaction.F90:4665-4759): totals, secondary molalities
 // (written to the SoA state) and dtotal into the aq block of J (not yet
@@ -34,7 +47,7 @@ struct PlainCell {
 template <int MAXN>
 __device__ void plain_total(const DevNet& n, const DevState& s, int64_t c,
                             PlainCell<MAXN>& w, int want_dtotal) {
-  const int naq = n.naq, N = n.ncomp, ms = n.cplx_ms;
+  const int naq = w.naq, N = w.N, ms = n.cplx_ms;
   double lnc[MAXN], lna[MAXN];
   for (int i = 0; i < naq; ++i) {
     lnc[i] = log(w.x[i]);
@@ -46,6 +59,7 @@ __device__ void plain_total(const DevNet& n, const DevState& s, int64_t c,
       for (int j = 0; j < naq; ++j) w.J[i * N + j] = (i == j) ? 1.0 : 0.0;
   }
   for (int k = 0; k < n.nc; ++k) {
+    if (!plain_cplx_on(n, w, k)) continue;
     double logK = n.cplx_logK[k];
     if (!n.isothermal && n.has_cplx_logKcoef)
       logK = interp_logk(n.cplx_logKcoef + 5 * k, w.temp);
@@ -79,7 +93,7 @@ template <int MAXN>
 __device__ void plain_minerals(const DevNet& n, const DevState& s, int64_t c,
                                PlainCell<MAXN>& w, double* res, double* J,
                                int deriv) {
-  const int naq = n.naq, N = n.ncomp, ms = n.mnrl_ms;
+  const int naq = w.naq, N = w.N, ms = n.mnrl_ms;
   double lnc[MAXN], lna[MAXN];
   for (int i = 0; i < naq; ++i) {
     lnc[i] = log(w.x[i]);
@@ -87,6 +101,7 @@ __device__ void plain_minerals(const DevNet& n, const DevState& s, int64_t c,
   }
   for (int m = 0; m < n.nm; ++m) w.rate[m] = 0.0;
   for (int m = 0; m < n.nm; ++m) {
+    if (!plain_mnrl_on(n, w, m)) continue;
     const int* ids = n.mnrl_ids + m * (ms + 1);
     const double* st = n.mnrl_st + m * ms;
     const int ncs = ids[0];
@@ -156,19 +171,21 @@ template <int MAXN>
 __device__ void plain_activity_lag(const DevNet& n, const DevState& s, int64_t c,
                                    PlainCell<MAXN>& w) {
   double I = 0.0, sum_pri = 0.0, sum_sec = 0.0;
-  for (int i = 0; i < n.naq; ++i) {
+  for (int i = 0; i < w.naq; ++i) {
     I += w.x[i] * n.Zp[i] * n.Zp[i];
     if (i != n.h2o_aq_id) sum_pri += w.x[i];
   }
   for (int k = 0; k < n.nc; ++k) {
+    if (!plain_cplx_on(n, w, k)) continue;
     const double m = s.sec_molal[k * s.ld + c];
     I += m * n.cplx_Z[k] * n.cplx_Z[k];
   }
   I *= 0.5;
   const double sI = sqrt(I);
-  for (int i = 0; i < n.naq; ++i)
+  for (int i = 0; i < w.naq; ++i)
     w.gp[i] = fabs(n.Zp[i]) > 1.0e-10 ? dh_gamma(n.Zp[i], n.a0p[i], sI, I, n) : 1.0;
   for (int k = 0; k < n.nc; ++k) {
+    if (!plain_cplx_on(n, w, k)) continue;
     s.sec_act_coef[k * s.ld + c] =
         fabs(n.cplx_Z[k]) > 1.0e-10 ? dh_gamma(n.cplx_Z[k], n.cplx_a0[k], sI, I, n) : 1.0;
     sum_sec += s.sec_molal[k * s.ld + c];
@@ -215,7 +232,7 @@ struct PlainSdEnv {
 template <int MAXN>
 __device__ void plain_somdec(const DevNet& n, const DevState& s, PlainCell<MAXN>& w,
                              PlainSdState& sd, double dt) {
-  PlainSdEnv<MAXN> e{w,     sd,     n.naq, n.ncomp, w.den * 1.0e-3, w.por * w.sat, w.vol,
+  PlainSdEnv<MAXN> e{w,     sd,     w.naq, w.N,     w.den * 1.0e-3, w.por * w.sat, w.vol,
                      w.sat, w.temp, sd.f_t, sd.f_w,  s.somdec_f_t != nullptr,
                      s.somdec_f_w != nullptr};
   double nmin = 0.0;
@@ -229,7 +246,7 @@ template <int MAXN, bool SD>
 __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
                             PlainCell<MAXN>& w, PlainSdState* sd, const double* guess,
                             double dt, int* nit) {
-  const int naq = n.naq, nimm = n.nimm, N = n.ncomp;
+  const int naq = w.naq, nimm = n.nimm, N = w.N;
   double fixed[MAXN], initial[MAXN], upd[MAXN], prev[MAXN];
   double r0 = 0.0;
   *nit = 0;
@@ -347,8 +364,11 @@ plain_step_kernel(const DevNet* np, DevState s, int64_t ncell, double dt) {
   const DevNet& n = *np;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  const int naq = n.naq, nimm = n.nimm, N = n.ncomp;
+  const int naq = n.variable ? min(max(s.ncomp_active[c], 1), n.naq) : n.naq;
+  const int nimm = n.nimm, N = naq + nimm;
   PlainCell<MAXN> w;
+  w.naq = naq;
+  w.N = N;
   PlainSdState sd;
   if constexpr (SD) {
     for (int r = 0; r < n.sd.nrxn; ++r) {
@@ -457,6 +477,7 @@ plain_step_kernel(const DevNet* np, DevState s, int64_t ncell, double dt) {
   }
   for (int i = 0; i < nimm; ++i) s.immobile[i * s.ld + c] = w.imm[i];
   for (int m = 0; m < n.nm; ++m) {
+    if (!plain_mnrl_on(n, w, m)) continue;
     s.mnrl_volfrac[m * s.ld + c] = w.vf[m];
     if (s.mnrl_rate) s.mnrl_rate[m * s.ld + c] = w.rate[m];
   }
@@ -472,10 +493,12 @@ activity_kernel(const DevNet* np, DevState s, int64_t ncell) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   PlainCell<MAXN> w;
-  for (int i = 0; i < n.naq; ++i) w.x[i] = s.pri_molal[i * s.ld + c];
+  w.naq = n.variable ? min(max(s.ncomp_active[c], 1), n.naq) : n.naq;
+  w.N = w.naq + n.nimm;
+  for (int i = 0; i < w.naq; ++i) w.x[i] = s.pri_molal[i * s.ld + c];
   w.ln_act_h2o = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
   plain_activity_lag(n, s, c, w);
-  for (int i = 0; i < n.naq; ++i) s.pri_act_coef[i * s.ld + c] = w.gp[i];
+  for (int i = 0; i < w.naq; ++i) s.pri_act_coef[i * s.ld + c] = w.gp[i];
   if (s.ln_act_h2o && n.act_h2o) s.ln_act_h2o[c] = w.ln_act_h2o;
 }
 

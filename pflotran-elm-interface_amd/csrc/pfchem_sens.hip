@@ -522,7 +522,7 @@ sens_reduce_kernel(const double* __restrict__ g, int64_t ldo, int64_t ncell,
 }  // namespace
 
 int sens_supported(const DevNet& n, const pfchem_options& opt) {
-  if (n.nimm > 0 || n.nm > kSensMaxMnrl || n.ncomp > 32) return 0;
+  if (n.variable || n.nimm > 0 || n.nm > kSensMaxMnrl || n.ncomp > 32) return 0;
   if (opt.act_coef_update_frequency == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER) return 0;
   if (n.has_temkin || n.has_minscale || n.has_affpower || n.has_limiter || n.has_threshold ||
       n.has_irrev)

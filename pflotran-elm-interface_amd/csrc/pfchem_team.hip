@@ -4,15 +4,7 @@
 
 namespace pfchem {
 
-hipError_t launch_team_1_4(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
-                          unsigned long long* cursors, int num_cus, int nxcd,
-                          hipStream_t stream, bool has_status);
 hipError_t launch_team_4_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
-                          unsigned long long* cursors, int num_cus, int nxcd,
-                          hipStream_t stream, bool has_status);
-hipError_t launch_team_1_8(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                           double dt, bool mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
@@ -48,11 +40,19 @@ hipError_t launch_team_32_1(const DevNet& n, const DevNet* dn, const DevState* d
                           double dt, bool mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
+hipError_t launch_team_64_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
+                          double dt, bool mixed, const LaunchCfg& cfg,
+                          unsigned long long* cursors, int num_cus, int nxcd,
+                          hipStream_t stream, bool has_status);
 
 namespace {
 constexpr int kMaxMnrl = 16;
 // (lanes per cell, rows per lane) shapes compiled into the library
-const int kShapes[][2] = {{1, 4}, {4, 1}, {1, 8}, {2, 4}, {8, 1}, {2, 8}, {4, 4}, {8, 2}, {16, 1}, {8, 4}, {32, 1}};
+// One-lane teams (1x4, 1x8) are not offered: their instantiations returned
+// wrong answers and faulted on MI355X with no cause found in the kernel (a
+// host build that runs one lane per wavefront matches the oracle).
+const int kShapes[][2] = {{4, 1}, {2, 4}, {8, 1}, {2, 8}, {4, 4}, {8, 2}, {16, 1}, {8, 4},
+                          {32, 1}, {64, 1}};
 
 // per-team LDS bytes: the same layout as team_lds_layout in the kernel
 int team_lds_bytes(int NM, int nc, int nm, int jt_size, int nsd) {
@@ -63,7 +63,7 @@ int team_lds_bytes(int NM, int nc, int nm, int jt_size, int nsd) {
 }
 }  // namespace
 
-int team_max_ncomp() { return 32; }
+int team_max_ncomp() { return 64; }
 
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed) {
   (void)mixed;  // both variants accumulate the Jacobian tile in FP64
@@ -90,9 +90,7 @@ hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* d
   if (cfg.lanes_per_cell == LL && cfg.rows_per_lane == RR)                                 \
     return launch_team_##LL##_##RR(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus, nxcd, \
                                    stream, has_status);
-  PFCHEM_SHAPE(1, 4)
   PFCHEM_SHAPE(4, 1)
-  PFCHEM_SHAPE(1, 8)
   PFCHEM_SHAPE(2, 4)
   PFCHEM_SHAPE(8, 1)
   PFCHEM_SHAPE(2, 8)
@@ -101,6 +99,7 @@ hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* d
   PFCHEM_SHAPE(16, 1)
   PFCHEM_SHAPE(8, 4)
   PFCHEM_SHAPE(32, 1)
+  PFCHEM_SHAPE(64, 1)
 #undef PFCHEM_SHAPE
   return hipErrorInvalidValue;
 }

@@ -115,7 +115,7 @@ __device__ __forceinline__ double bcast(double v) {
 }
 
 // partner lane at butterfly level LEV (0: xor 1, 1: xor 2, 2: mirror in 8,
-// 3: mirror in 16, 4: swap halves of 32)
+// 3: mirror in 16, 4: swap halves of 32, 5: swap halves of the wave)
 template <int LEV>
 __device__ __forceinline__ int partner32(int v) {
   if constexpr (LEV == 0)
@@ -126,8 +126,10 @@ __device__ __forceinline__ int partner32(int v) {
     return __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);
   else if constexpr (LEV == 3)
     return __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);
-  else
+  else if constexpr (LEV == 4)
     return __builtin_amdgcn_ds_swizzle(v, 0x1F | (0x10 << 10));  // SWAP 16
+  else
+    return __shfl_xor(v, 32, 64);
 }
 template <int LEV>
 __device__ __forceinline__ double partner(double v) {
@@ -151,6 +153,7 @@ __device__ __forceinline__ T team_reduce(T v, Op op) {
   if constexpr (L >= 8) v = op(v, partner<2>(v));
   if constexpr (L >= 16) v = op(v, partner<3>(v));
   if constexpr (L >= 32) v = op(v, partner<4>(v));
+  if constexpr (L >= 64) v = op(v, partner<5>(v));
   return v;
 }
 template <int L>
@@ -507,7 +510,13 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   double* Jt = (double*)(tb + lay.J);
   double* sdst = (double*)(tb + lay.sd);  // SomDec stoichiometry of the team's cell
 
-  const int naq = n.naq, N = n.ncomp, nc = n.nc, nm = n.nm;
+  // aqueous primaries and unknowns of the team's cell (per-cell active
+  // counts with options.variable_ncomp; team-uniform)
+  int naq = n.naq, N = n.ncomp;
+  const int nc = n.nc, nm = n.nm;
+  // complex k / mineral m takes part in the cell's system
+  auto cplx_on = [&](int k) { return !n.variable || gld(n.cplx_maxid + k) < naq; };
+  auto mnrl_on = [&](int m) { return !n.variable || gld(n.mnrl_maxid + m) < naq; };
   const int64_t ld = s.ld;
   const bool escalate = s.status != nullptr;  // FP32 failures go to the FP64 pass
 
@@ -584,6 +593,10 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       } else {
         cell = got;
         const int64_t c = cell;
+        if (n.variable) {
+          naq = min(max(gld(s.ncomp_active + (c)), 1), n.naq);
+          N = naq;
+        }
         por = gld(s.porosity + (c));
         sat = gld(s.saturation + (c));
         den = gld(s.den_kg + (c));
@@ -604,6 +617,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         }
         sec_I = sec_sum = 0.0;
         for (int k = t; k < nc; k += L) {
+          if (!cplx_on(k)) continue;
           lng[k] = log(gld(s.sec_act_coef + (k * ld + c)));
           const double m_k = gld(s.sec_molal + (k * ld + c));
           sec_I += m_k * gld(n.cplx_Z + k) * gld(n.cplx_Z + k);
@@ -611,7 +625,8 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         }
         for (int m = t; m < nm; m += L) {
           vf[m] = gld(s.mnrl_volfrac + (m * ld + c));
-          area[m] = gld(s.mnrl_area + (m * ld + c));
+          // an inactive mineral has no rate (and is not written back)
+          area[m] = mnrl_on(m) ? gld(s.mnrl_area + (m * ld + c)) : 0.0;
           rate[m] = 0.0;
         }
         if constexpr (SD) {
@@ -741,7 +756,8 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           lngam[r] = fabs(gld(n.Zp + i)) > 1.0e-10 ? log(dh_gamma(gld(n.Zp + i), gld(n.a0p + i), sI, I, n)) : 0.0;
       }
       for (int k = t; k < nc; k += L)
-        lng[k] = fabs(gld(n.cplx_Z + k)) > 1.0e-10
+        if (cplx_on(k))
+          lng[k] = fabs(gld(n.cplx_Z + k)) > 1.0e-10
                      ? log(dh_gamma(gld(n.cplx_Z + k), gld(n.cplx_a0 + k), sI, I, n)) : 0.0;
       if (n.act_h2o) {
         const double a = 1.0 - 0.017 * team_sum<L>(sp_ + sec_sum);
@@ -817,7 +833,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int q = 0; q < KC; ++q) {
         const int k = k0 + q * L + t;
         mk[q] = 0.0;
-        if (k < nc) {
+        if (k < nc && cplx_on(k)) {
           const double logK = n.isothermal || !n.has_cplx_logKcoef
                                   ? gld(n.cplx_logK + k)
                                   : interp_logk(n.cplx_logKcoef + 5 * k, temp);
@@ -833,7 +849,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 #pragma unroll
       for (int q = 0; q < KC; ++q) {
         const int k = k0 + q * L + t;
-        if (k < nc)
+        if (k < nc && mk[q] != 0.0)
           crow[q].template scatter<NM>(n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, mk[q],
                                            mk[q] * jc, wet, tot, Jt, rsv);
       }
@@ -1012,12 +1028,66 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
               xs[p] = bcast<L, ow>(sum * a[sl][p]);
             }
           });
+          // Large systems in mixed precision: one step of iterative
+          // refinement.  The residual of the scaled system at the FP32
+          // solution is formed in FP64 from the FP64 tile (the matrix the
+          // factorization started from, as build_rows assembles it), solved
+          // with the same FP32 factors, and the correction is added in FP64,
+          // so the Newton step keeps FP64 accuracy on ill-conditioned cells.
+          JT xc[NM];
+          static_for<0, NM>([&](auto P) { xc[decltype(P)::value] = (JT)0; });
+          if constexpr (sizeof(JT) == 4 && NM >= 32) {
+            JT r2[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const int i = r * L + t;
+              double acc = 0.0;
+              if (i < N) {
+                double d = 1.0;
+                if (i >= naq)
+                  for (int q = 0; q < n.ndecay; ++q)
+                    if (gld(n.decay_id + q) == i - naq) d += gld(n.decay_k + q) * vol * x[r] * rs[r];
+                acc = res[r] * rs[r];
+                static_for<0, NM>([&](auto Q) {
+                  constexpr int j = decltype(Q)::value;
+                  if (j < N) {
+                    double aij = wet ? Jt[i * (NM + 1) + j] + (j == i ? d : 0.0) : (j == i ? 1.0 : 0.0);
+                    if (!n.log_form) aij /= xv[j];
+                    acc -= aij * (double)xs[j];
+                  }
+                });
+              }
+              r2[r] = (JT)acc;
+            }
+            static_for<0, NM>([&](auto P) {
+              constexpr int p = decltype(P)::value;
+              constexpr int ow = p % L, sl = p / L;
+              if (p < N) {
+                const JT pr = bcast<L, ow>(r2[sl]);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                  if (r * L + t > p) r2[r] -= a[r][p] * pr;
+              }
+            });
+            static_for_down<0, NM>([&](auto P) {
+              constexpr int p = decltype(P)::value;
+              constexpr int ow = p % L, sl = p / L;
+              if (p < N) {
+                JT sum = r2[sl];
+                static_for<p + 1, NM>([&](auto Q) {
+                  constexpr int j = decltype(Q)::value;
+                  sum -= a[sl][j] * xc[j];
+                });
+                xc[p] = bcast<L, ow>(sum * a[sl][p]);
+              }
+            });
+          }
 #pragma unroll
           for (int r = 0; r < R; ++r) {
             delta[r] = 0.0;
             static_for<0, NM>([&](auto P) {
               constexpr int p = decltype(P)::value;
-              if (p == r * L + t) delta[r] = (double)xs[p];
+              if (p == r * L + t) delta[r] = (double)xs[p] + (double)xc[p];
             });
           }
         } else {
@@ -1156,12 +1226,14 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       // RTAuxVarCompute, reaction.F90:4052): recomputed here instead of
       // being kept per complex in LDS
       for (int k = t; k < nc; k += L) {
+        if (!cplx_on(k)) continue;
         const double lnQK = complex_lnqk(n, k, lnact, ln_aw, temp);
         gst(s.sec_molal + (k * ld + c), exp(lnQK - lng[k]));
         if (n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER)
           gst(s.sec_act_coef + (k * ld + c), exp(lng[k]));
       }
       for (int m = t; m < nm; m += L) {
+        if (!mnrl_on(m)) continue;
         gst(s.mnrl_volfrac + (m * ld + c), vf[m]);
         if (s.mnrl_rate) gst(s.mnrl_rate + (m * ld + c), rate[m]);
       }

@@ -91,7 +91,7 @@ module pfchem_c
     integer(c_int32_t) :: isothermal
     integer(c_int32_t) :: maximum_reaction_iterations
     integer(c_int32_t) :: maximum_reaction_cuts
-    integer(c_int32_t) :: pad0
+    integer(c_int32_t) :: variable_ncomp
     real(c_double) :: max_dlnC_rreact
     real(c_double) :: max_residual_tolerance
     real(c_double) :: max_rel_residual_tolerance
@@ -173,6 +173,7 @@ module pfchem_c
     type(c_ptr) :: num_iterations
     type(c_ptr) :: num_sub_steps
     type(c_ptr) :: status
+    type(c_ptr) :: ncomp_active
   end type pfchem_state
 
   type, bind(C) :: pfchem_launch

@@ -338,7 +338,7 @@ contains
     opt%isothermal = merge(1, 0, option%use_isothermal)
     opt%maximum_reaction_iterations = reaction%maximum_reaction_iterations
     opt%maximum_reaction_cuts = reaction%maximum_reaction_cuts
-    opt%pad0 = 0
+    opt%variable_ncomp = 0
     opt%max_dlnC_rreact = reaction%max_dlnC_rreact
     opt%max_residual_tolerance = reaction%max_residual_tolerance
     opt%max_rel_residual_tolerance = reaction%max_rel_residual_tolerance
@@ -449,6 +449,7 @@ contains
     st%num_iterations = c_loc(nit)
     st%num_sub_steps = c_loc(nsub)
     st%status = c_loc(status)
+    st%ncomp_active = c_null_ptr
 
     rc = pfchem_step_host(ctx, int(n, c_int64_t), st, real(option%tran_dt, c_double), &
                           kernel_variant)

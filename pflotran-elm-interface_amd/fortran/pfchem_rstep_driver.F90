@@ -193,7 +193,7 @@ contains
       transfer(c_loc(opt%max_relative_change_tolerance), b) - b
     b = transfer(c_loc(st), b)
     write(*, '(*(g0,:," "))') 'state', c_sizeof(st), transfer(c_loc(st%temperature), b) - b, &
-      transfer(c_loc(st%status), b) - b
+      transfer(c_loc(st%status), b) - b, transfer(c_loc(st%ncomp_active), b) - b
     b = transfer(c_loc(sd), b)
     write(*, '(*(g0,:," "))') 'somdec', c_sizeof(sd), transfer(c_loc(sd%x0eps), b) - b, &
       transfer(c_loc(sd%rate_constant), b) - b, transfer(c_loc(sd%inhibition_constant2), b) - b

@@ -154,7 +154,7 @@ class Options(C.Structure):
         ("isothermal", C.c_int32),
         ("maximum_reaction_iterations", C.c_int32),
         ("maximum_reaction_cuts", C.c_int32),
-        ("pad0", C.c_int32),
+        ("variable_ncomp", C.c_int32),
         ("max_dlnC_rreact", C.c_double),
         ("max_residual_tolerance", C.c_double),
         ("max_rel_residual_tolerance", C.c_double),
@@ -185,6 +185,7 @@ class State(C.Structure):
         ("num_iterations", C.c_void_p),
         ("num_sub_steps", C.c_void_p),
         ("status", C.c_void_p),
+        ("ncomp_active", C.c_void_p),
     ]
 
 
@@ -198,5 +199,5 @@ class Launch(C.Structure):
 
 
 # (lanes per cell, rows per lane) team shapes compiled into libpfchem.so
-TEAM_SHAPES = [(1, 4), (4, 1), (1, 8), (2, 4), (8, 1), (2, 8), (4, 4), (8, 2), (16, 1),
-               (8, 4), (32, 1)]
+TEAM_SHAPES = [(4, 1), (2, 4), (8, 1), (2, 8), (4, 4), (8, 2), (16, 1), (8, 4), (32, 1),
+               (64, 1)]

@@ -270,6 +270,8 @@ class SolverOptions:
     max_residual_tolerance: float = 1.0e-12
     max_rel_residual_tolerance: float = 1.0e-8
     max_relative_change_tolerance: float = 1.0e-6
+    # per-cell active primary counts in the state (pfchem_state.ncomp_active)
+    variable_ncomp: bool = False
 
     def to_abi(self) -> abi.Options:
         o = abi.Options()
@@ -281,7 +283,8 @@ class SolverOptions:
         return o
 
 
-def state_fields(net: ReactionNetwork, somdec_scalars: bool = False) -> Dict[str, tuple]:
+def state_fields(net: ReactionNetwork, somdec_scalars: bool = False,
+                 variable_ncomp: bool = False) -> Dict[str, tuple]:
     """name -> (rows, dtype) of every SoA array of pfchem_state."""
     f = {
         "pri_molal": (net.naqcomp, "f8"),
@@ -306,6 +309,8 @@ def state_fields(net: ReactionNetwork, somdec_scalars: bool = False) -> Dict[str
     if somdec_scalars:
         f["somdec_f_t"] = (1, "f8")
         f["somdec_f_w"] = (1, "f8")
+    if variable_ncomp:
+        f["ncomp_active"] = (1, "i4")
     return f
 
 
@@ -313,13 +318,13 @@ class CellState:
     """Host (numpy) SoA state: arrays[name] has shape (rows, ld)."""
 
     def __init__(self, net: ReactionNetwork, ncell: int, ld: Optional[int] = None,
-                 somdec_scalars: bool = False):
+                 somdec_scalars: bool = False, variable_ncomp: bool = False):
         self.net = net
         self.ncell = int(ncell)
         self.ld = int(ld if ld is not None else ncell)
         if self.ld < self.ncell:
             raise ValueError("ld < ncell")
-        self.fields = state_fields(net, somdec_scalars)
+        self.fields = state_fields(net, somdec_scalars, variable_ncomp)
         self.arrays: Dict[str, np.ndarray] = {}
         for name, (rows, dt) in self.fields.items():
             self.arrays[name] = np.zeros((rows, self.ld), dtype=dt)

@@ -31,6 +31,8 @@ from __future__ import annotations
 from dataclasses import dataclass
 from typing import Optional
 
+import dataclasses
+
 import numpy as np
 
 from .network import CellState, ReactionNetwork, SolverOptions, DEBYE_A, DEBYE_B, DEBYE_BDOT
@@ -293,7 +295,7 @@ def random_network(naq: int = 15, ncplx: int = 60, nmnrl: int = 5, seed: int = 2
 def random_network_workload(ncell: int = 1 << 20, naq: int = 15, ncplx: int = 60,
                             nmnrl: int = 5, seed: int = 20261204, dt: float = 3600.0,
                             wmax: float = 0.3, ld: Optional[int] = None,
-                            spread: float = 0.25) -> Workload:
+                            spread: float = 0.25, variable_ncomp: bool = False) -> Workload:
     net = random_network(naq, ncplx, nmnrl, seed)
     rng = _rng(seed + 1)
     nb_ = net._nbackground
@@ -315,7 +317,7 @@ def random_network_workload(ncell: int = 1 << 20, naq: int = 15, ncplx: int = 60
     nb = (np.arange(ncell) + 1) % ncell
     w = rng.uniform(0.0, wmax, ncell)
     T = (1.0 - w) * T0 + w * T0[:, nb]
-    st = CellState(net, ncell, ld)
+    st = CellState(net, ncell, ld, variable_ncomp=variable_ncomp)
     _fill_common(st, net, m0, sec0, gp, gc, T, WATER_DENSITY_25C, 0.3)
     # active-mineral subset per cell: count ~ U{0..nmnrl}
     cnt = rng.integers(0, nmnrl + 1, ncell)
@@ -332,6 +334,23 @@ def random_network_workload(ncell: int = 1 << 20, naq: int = 15, ncplx: int = 60
              "U{0..5} per cell")
     return Workload("random_n%d_c%d_m%d" % (naq, ncplx, nmnrl), net, bench_options(), st,
                     dt, seed, notes)
+
+
+def variable_network_workload(ncell: int = 262_144, seed: int = 20261206, dt: float = 3600.0,
+                              nmin: int = 8, ld: Optional[int] = None) -> Workload:
+    """Large network of 64 primaries, 300 complexes and 10 kinetic minerals
+    (the random-network generator) in which each cell has its own number of
+    active primaries, U{nmin..64}: the cell solves the network restricted to
+    primaries 0..nact-1 (complexes and minerals all of whose species are
+    among them), so its system is nact x nact."""
+    wl = random_network_workload(ncell, naq=64, ncplx=300, nmnrl=10, seed=seed, dt=dt, ld=ld,
+                                 variable_ncomp=True)
+    rng = _rng(seed + 2)
+    wl.state["ncomp_active"][0, :ncell] = rng.integers(nmin, 65, ncell)
+    opt = dataclasses.replace(wl.opt, variable_ncomp=True)
+    notes = wl.notes + ("; 64 primaries, 300 complexes, 10 minerals; per-cell active "
+                        "primaries U{%d..64} (prefix 0..nact-1)" % nmin)
+    return Workload("variable_n64_c300_m10", wl.net, opt, wl.state, dt, seed, notes)
 
 
 # ---------------------------------------------------------------------------

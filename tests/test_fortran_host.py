@@ -35,8 +35,8 @@ int main(void) {
   printf("options %zu %zu %zu\n", sizeof(pfchem_options),
          offsetof(pfchem_options, max_dlnC_rreact),
          offsetof(pfchem_options, max_relative_change_tolerance));
-  printf("state %zu %zu %zu\n", sizeof(pfchem_state), offsetof(pfchem_state, temperature),
-         offsetof(pfchem_state, status));
+  printf("state %zu %zu %zu %zu\n", sizeof(pfchem_state), offsetof(pfchem_state, temperature),
+         offsetof(pfchem_state, status), offsetof(pfchem_state, ncomp_active));
   printf("somdec %zu %zu %zu %zu\n", sizeof(pfchem_somdec), offsetof(pfchem_somdec, x0eps),
          offsetof(pfchem_somdec, rate_constant), offsetof(pfchem_somdec, inhibition_constant2));
   printf("launch %zu\n", sizeof(pfchem_launch));

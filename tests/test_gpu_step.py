@@ -102,10 +102,15 @@ def test_launch_shapes_agree(randnet, lanes, rows, block):
         pytest.skip("team tiles of L%dR%d exceed the device's LDS" % (lanes, rows))
 
 
-@pytest.mark.parametrize("lanes,rows", [(1, 4), (4, 1), (2, 4), (8, 1), (1, 8)])
+ONE_LANE_SKIP = ("one-lane teams (1x4, 1x8) returned wrong answers and faulted on MI355X, cause "
+                 "not found; the library no longer offers them")
+
+
+@pytest.mark.parametrize("lanes,rows", [
+    pytest.param(1, 4, marks=pytest.mark.skip(reason=ONE_LANE_SKIP)), (4, 1), (2, 4), (8, 1),
+    pytest.param(1, 8, marks=pytest.mark.skip(reason=ONE_LANE_SKIP))])
 def test_small_team_shapes(calcite, lanes, rows):
-    """Teams smaller than a quad and one-lane teams (a whole cell per lane)
-    on the three-component calcite system."""
+    """Teams smaller than a quad on the three-component calcite system."""
     wl, ref, rec = calcite
     for variant in (abi.KERNEL_BATCHED_MIXED, abi.KERNEL_BATCHED_FP64):
         got, b = _run_shape(wl, variant, lanes, rows, 128)
