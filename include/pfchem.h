@@ -323,7 +323,7 @@ int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
    team of lanes_per_cell lanes owns one cell (64 / lanes_per_cell cells per
    wavefront) and each lane owns rows_per_lane rows of the cell's system
    (lanes_per_cell * rows_per_lane >= ncomp).  Shapes compiled in:
-   4x1 2x4 8x1 2x8 4x4 8x2 16x1 8x4 32x1 64x1. */
+   4x1 2x4 2x8 4x4 8x2 16x1 8x4 32x1 64x1. */
 typedef struct pfchem_launch {
   int32_t lanes_per_cell;   /* 2, 4, 8, 16, 32 or 64                      */
   int32_t block_size;       /* threads per workgroup (64, 128 or 256)     */

@@ -489,13 +489,15 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   int L = 32, R = 1;
   if (d.ncomp > 32) {
     L = 64;  // one cell per wavefront
+  } else if (d.variable) {
+    if (d.ncomp <= 16) L = 16;  // per-cell active counts: 16x1, 32x1, 64x1
   } else if (d.has_somdec) {
     if (d.ncomp <= 16) L = 16;  // SomDec kernels exist for 16x1, 8x2, 32x1
   } else if (d.ncomp <= 4) {
     L = 4;
     R = 1;
   } else if (d.ncomp <= 8) {
-    L = 8;
+    L = 16;
     R = 1;
   } else if (d.ncomp <= 16) {
     L = 16;

@@ -30,6 +30,10 @@ int team_max_ncomp();
 constexpr bool team_has_somdec(int L, int R) {
   return (L == 16 && R == 1) || (L == 8 && R == 2) || (L == 32 && R == 1);
 }
+// team shapes with an instantiation for per-cell active primary counts
+constexpr bool team_has_variable(int L, int R) {
+  return R == 1 && (L == 16 || L == 32 || L == 64);
+}
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
 // dn / ds: device copies of the network / state descriptors.  The mixed
 // variant runs the FP32 pass and then an FP64 pass over the cells the FP32

@@ -474,9 +474,10 @@ enum Outcome : int { kContinue = 0, kConverged = 1, kFailMaxIt = 2, kFailSingula
 #define PFCHEM_TEAM_WAVES_PER_EU 2
 #endif
 
-// SD: the network carries the SomDec sandbox (instantiated for a few team
-// shapes only, so the other networks' kernels keep their register budget)
-template <int L, int R, typename JT, bool SD>
+// SD: the network carries the SomDec sandbox; VAR: cells carry their own
+// active primary counts (options.variable_ncomp).  Both are instantiated for
+// a few team shapes only, so the other networks' kernels keep their code.
+template <int L, int R, typename JT, bool SD, bool VAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PFCHEM_TEAM_WAVES_PER_EU)))
 team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                  unsigned long long* __restrict__ cursors, int nxcd, int rescue_only) {
@@ -510,13 +511,16 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   double* Jt = (double*)(tb + lay.J);
   double* sdst = (double*)(tb + lay.sd);  // SomDec stoichiometry of the team's cell
 
-  // aqueous primaries and unknowns of the team's cell (per-cell active
-  // counts with options.variable_ncomp; team-uniform)
-  int naq = n.naq, N = n.ncomp;
+  // aqueous primaries and unknowns of the team's cell: the network's, or
+  // with VAR the cell's active count (team-uniform)
+  const int naq0 = n.naq, N0 = n.ncomp;
+  int naq_var = naq0, N_var = N0;
+  const int& naq = VAR ? naq_var : naq0;
+  const int& N = VAR ? N_var : N0;
   const int nc = n.nc, nm = n.nm;
   // complex k / mineral m takes part in the cell's system
-  auto cplx_on = [&](int k) { return !n.variable || gld(n.cplx_maxid + k) < naq; };
-  auto mnrl_on = [&](int m) { return !n.variable || gld(n.mnrl_maxid + m) < naq; };
+  auto cplx_on = [&](int k) { return !VAR || gld(n.cplx_maxid + k) < naq; };
+  auto mnrl_on = [&](int m) { return !VAR || gld(n.mnrl_maxid + m) < naq; };
   const int64_t ld = s.ld;
   const bool escalate = s.status != nullptr;  // FP32 failures go to the FP64 pass
 
@@ -593,9 +597,9 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       } else {
         cell = got;
         const int64_t c = cell;
-        if (n.variable) {
-          naq = min(max(gld(s.ncomp_active + (c)), 1), n.naq);
-          N = naq;
+        if constexpr (VAR) {
+          naq_var = min(max(gld(s.ncomp_active + (c)), 1), n.naq);
+          N_var = naq_var;
         }
         por = gld(s.porosity + (c));
         sat = gld(s.saturation + (c));
@@ -849,7 +853,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 #pragma unroll
       for (int q = 0; q < KC; ++q) {
         const int k = k0 + q * L + t;
-        if (k < nc && mk[q] != 0.0)
+        if (k < nc && (!VAR || mk[q] != 0.0))
           crow[q].template scatter<NM>(n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, mk[q],
                                            mk[q] * jc, wet, tot, Jt, rsv);
       }
@@ -1254,7 +1258,7 @@ __global__ void zero_cursors(unsigned long long* c, int n) {
   if ((int)threadIdx.x < n) c[16 * threadIdx.x] = 0ull;
 }
 
-template <int L, int R, typename JT, bool SD>
+template <int L, int R, typename JT, bool SD, bool VAR>
 hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                      double dt, const LaunchCfg& cfg, unsigned long long* cursors,
                      int num_cus, int nxcd, hipStream_t stream, int rescue_only) {
@@ -1269,14 +1273,33 @@ hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int6
   if (blocks < 1) blocks = 1;
   if (lds > 65536) {
     const hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&team_step_kernel<L, R, JT, SD>),
+        reinterpret_cast<const void*>(&team_step_kernel<L, R, JT, SD, VAR>),
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   zero_cursors<<<1, 64, 0, stream>>>(cursors, nxcd);
-  team_step_kernel<L, R, JT, SD><<<dim3((unsigned)blocks), dim3(block), lds, stream>>>(
+  team_step_kernel<L, R, JT, SD, VAR><<<dim3((unsigned)blocks), dim3(block), lds, stream>>>(
       dn, ds, ncell, dt, cursors, nxcd, rescue_only);
   return hipGetLastError();
+}
+
+// FP64 variant, or the mixed variant (FP32 pass, then an FP64 pass over the
+// cells the FP32 pass handed over)
+template <int L, int R, bool SD, bool VAR>
+hipError_t launch_variant(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
+                          double dt, bool mixed, const LaunchCfg& cfg,
+                          unsigned long long* cursors, int num_cus, int nxcd,
+                          hipStream_t stream, bool has_status) {
+  if (!mixed)
+    return launch_LR<L, R, double, SD, VAR>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd,
+                                            stream, 0);
+  hipError_t e = launch_LR<L, R, float, SD, VAR>(n, dn, ds, ncell, dt, cfg, cursors, num_cus,
+                                                 nxcd, stream, 0);
+  if (e != hipSuccess || !has_status) return e;
+  LaunchCfg c64 = cfg;
+  while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
+  return launch_LR<L, R, double, SD, VAR>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd,
+                                          stream, 1);
 }
 
 template <int L, int R>
@@ -1284,30 +1307,18 @@ hipError_t launch_pair(const DevNet& n, const DevNet* dn, const DevState* ds, in
                        double dt, bool mixed, const LaunchCfg& cfg,
                        unsigned long long* cursors, int num_cus, int nxcd,
                        hipStream_t stream, bool has_status) {
-  if (!n.has_somdec) {
-    if (!mixed)
-      return launch_LR<L, R, double, false>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd,
-                                            stream, 0);
-    hipError_t e = launch_LR<L, R, float, false>(n, dn, ds, ncell, dt, cfg, cursors, num_cus,
-                                                 nxcd, stream, 0);
-    if (e != hipSuccess || !has_status) return e;
-    LaunchCfg c64 = cfg;
-    while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
-    return launch_LR<L, R, double, false>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd,
-                                          stream, 1);
+  if (n.variable) {
+    if constexpr (team_has_variable(L, R))
+      return launch_variant<L, R, false, true>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                               nxcd, stream, has_status);
+    return hipErrorInvalidValue;
   }
-  if constexpr (team_has_somdec(L, R)) {
-    if (!mixed)
-      return launch_LR<L, R, double, true>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd,
-                                           stream, 0);
-    hipError_t e = launch_LR<L, R, float, true>(n, dn, ds, ncell, dt, cfg, cursors, num_cus,
-                                                nxcd, stream, 0);
-    if (e != hipSuccess || !has_status) return e;
-    LaunchCfg c64 = cfg;
-    while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
-    return launch_LR<L, R, double, true>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd,
-                                         stream, 1);
-  }
+  if (!n.has_somdec)
+    return launch_variant<L, R, false, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                              nxcd, stream, has_status);
+  if constexpr (team_has_somdec(L, R))
+    return launch_variant<L, R, true, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                             nxcd, stream, has_status);
   return hipErrorInvalidValue;
 }
 

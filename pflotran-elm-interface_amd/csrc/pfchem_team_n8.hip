@@ -12,12 +12,4 @@ hipError_t launch_team_2_4(const DevNet& n, const DevNet* dn, const DevState* ds
                            has_status);
 }
 
-hipError_t launch_team_8_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
-                          unsigned long long* cursors, int num_cus, int nxcd,
-                          hipStream_t stream, bool has_status) {
-  return launch_pair<8, 1>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus, nxcd, stream,
-                           has_status);
-}
-
 }  // namespace pfchem

@@ -199,5 +199,4 @@ class Launch(C.Structure):
 
 
 # (lanes per cell, rows per lane) team shapes compiled into libpfchem.so
-TEAM_SHAPES = [(4, 1), (2, 4), (8, 1), (2, 8), (4, 4), (8, 2), (16, 1), (8, 4), (32, 1),
-               (64, 1)]
+TEAM_SHAPES = [(4, 1), (2, 4), (2, 8), (4, 4), (8, 2), (16, 1), (8, 4), (32, 1), (64, 1)]

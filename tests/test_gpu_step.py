@@ -104,10 +104,13 @@ def test_launch_shapes_agree(randnet, lanes, rows, block):
 
 ONE_LANE_SKIP = ("one-lane teams (1x4, 1x8) returned wrong answers and faulted on MI355X, cause "
                  "not found; the library no longer offers them")
+EIGHT_SKIP = ("8x1 teams did not finish on the calcite cells on MI355X, cause not found; the "
+              "library no longer offers the shape")
 
 
 @pytest.mark.parametrize("lanes,rows", [
-    pytest.param(1, 4, marks=pytest.mark.skip(reason=ONE_LANE_SKIP)), (4, 1), (2, 4), (8, 1),
+    pytest.param(1, 4, marks=pytest.mark.skip(reason=ONE_LANE_SKIP)), (4, 1), (2, 4),
+    pytest.param(8, 1, marks=pytest.mark.skip(reason=EIGHT_SKIP)),
     pytest.param(1, 8, marks=pytest.mark.skip(reason=ONE_LANE_SKIP))])
 def test_small_team_shapes(calcite, lanes, rows):
     """Teams smaller than a quad on the three-component calcite system."""
