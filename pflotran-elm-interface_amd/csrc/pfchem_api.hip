@@ -26,6 +26,10 @@ struct pfchem_ctx {
   DevState* hstate_ring = nullptr;        // pinned host mirror the slots are copied from
   int ring_next = 0;
   LaunchCfg launch[3] = {{1, 1, 128, 0}, {16, 1, 256, 4}, {16, 1, 256, 4}};
+  // per-cell active counts: team shape per size bucket, per batched variant
+  LaunchCfg vlaunch[2][kBuckets];
+  int32_t* buckets = nullptr;  // kBuckets lists of ncell entries, then the counts
+  int64_t bucket_cap = 0;
   pfchem_network net{};   // sizes only (host pointers are not kept)
   pfchem_options opt{};
   bool has_somdec = false;
@@ -508,6 +512,13 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     while (c.block_size > 64 && !team_supports(d, c, v == PFCHEM_KERNEL_BATCHED_MIXED))
       c.block_size /= 2;
     ctx->launch[v] = c;
+    // size buckets of per-cell active counts: 16x1, 32x1, 64x1 teams
+    for (int b = 0; b < kBuckets; ++b) {
+      LaunchCfg cb{kBucketBound[b], 1, 256, 4, ctx->max_lds};
+      while (cb.block_size > 64 && !team_supports(d, cb, v == PFCHEM_KERNEL_BATCHED_MIXED))
+        cb.block_size /= 2;
+      ctx->vlaunch[v - 1][b] = cb;
+    }
   }
   if (hipMalloc(&ctx->dnet_dev, sizeof(DevNet)) != hipSuccess ||
       hipMalloc(&ctx->dstate_ring, sizeof(DevState) * kStateRing) != hipSuccess ||
@@ -526,6 +537,7 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->dbuf) (void)hipFree(ctx->dbuf);
   if (ctx->counter) (void)hipFree(ctx->counter);
   if (ctx->stage) (void)hipFree(ctx->stage);
+  if (ctx->buckets) (void)hipFree(ctx->buckets);
   if (ctx->dnet_dev) (void)hipFree(ctx->dnet_dev);
   if (ctx->dstate_ring) (void)hipFree(ctx->dstate_ring);
   if (ctx->hstate_ring) (void)hipHostFree(ctx->hstate_ring);
@@ -572,6 +584,8 @@ static int to_dev_state(pfchem_ctx* ctx, const pfchem_state* st, int64_t ncell, 
   d->somdec_f_t = st->somdec_f_t;
   d->somdec_f_w = st->somdec_f_w;
   d->ncomp_active = n.variable ? st->ncomp_active : nullptr;
+  d->cell_list = nullptr;
+  d->list_count = nullptr;
   d->num_iterations = st->num_iterations;
   d->num_sub_steps = st->num_sub_steps;
   d->status = st->status;
@@ -602,6 +616,41 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
     e = launch_plain_step(ctx->dnet, ctx->dnet_dev, d, ncell, dt, s);
   } else {
     const bool mixed = variant == PFCHEM_KERNEL_BATCHED_MIXED;
+    if (ctx->dnet.variable) {
+      // per-cell active counts: cells sorted into size buckets on the
+      // device, one launch per bucket with the bucket's team shape
+      if (ncell > 0x7fffffff) return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "variable_ncomp: ncell >= 2^31");
+      if (ncell > ctx->bucket_cap) {
+        if (ctx->buckets) (void)hipFree(ctx->buckets);
+        ctx->buckets = nullptr;
+        ctx->bucket_cap = 0;
+        if (hipMalloc(&ctx->buckets, sizeof(int32_t) * ((size_t)kBuckets * ncell + 64)) != hipSuccess)
+          return fail(ctx, PFCHEM_ERR_NOMEM, "bucket lists");
+        ctx->bucket_cap = ncell;
+      }
+      int32_t* counts = ctx->buckets + (size_t)kBuckets * ncell;
+      e = launch_buckets(d.ncomp_active, ncell, ctx->dnet.naq, ctx->buckets, counts, s);
+      if (e != hipSuccess) return hip_fail(ctx, e, "bucket launch");
+      for (int b = 0; b < kBuckets; ++b) {
+        if (b > 0 && kBucketBound[b - 1] >= ctx->dnet.naq) break;  // no cell that large
+        const LaunchCfg& cb = ctx->vlaunch[variant - 1][b];
+        if (!team_supports(ctx->dnet, cb, mixed))
+          return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "bucket launch configuration does not fit");
+        DevState db = d;
+        db.cell_list = ctx->buckets + (size_t)b * ncell;
+        db.list_count = counts + b;
+        DevState* slot = ctx->dstate_ring + ctx->ring_next;
+        DevState* hslot = ctx->hstate_ring + ctx->ring_next;
+        ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
+        *hslot = db;
+        e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+        e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cb, ctx->counter,
+                             ctx->num_cus, ctx->nxcd, s, d.status != nullptr);
+        if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+      }
+      return PFCHEM_OK;
+    }
     const LaunchCfg& cfg = ctx->launch[variant];
     if (!team_supports(ctx->dnet, cfg, mixed))
       return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
@@ -652,6 +701,9 @@ int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg) {
       !team_supports(ctx->dnet, c, variant == PFCHEM_KERNEL_BATCHED_MIXED))
     return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
   ctx->launch[variant] = c;
+  if (ctx->dnet.variable && variant != PFCHEM_KERNEL_PLAIN_FP64 && c.rows_per_lane == 1)
+    for (int b = 0; b < kBuckets; ++b)
+      if (kBucketBound[b] == c.lanes_per_cell) ctx->vlaunch[variant - 1][b] = c;
   return PFCHEM_OK;
 }
 

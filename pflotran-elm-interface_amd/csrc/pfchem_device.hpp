@@ -142,6 +142,11 @@ struct DevState {
   int32_t* num_iterations;
   int32_t* num_sub_steps;
   int32_t* status;
+  // size bucket of a launch over cells with per-cell active counts: the
+  // launch works on cells cell_list[0 .. *list_count) (device memory filled
+  // by the bucketing kernel); NULL: cells 0 .. ncell
+  const int* cell_list;
+  const int* list_count;
 };
 
 __device__ __forceinline__ double sat_of(const DevState& s, int64_t c) {

@@ -43,6 +43,14 @@ hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* d
                             unsigned long long* cursors, int num_cus, int nxcd,
                             hipStream_t stream, bool has_status);
 
+// size buckets of cells with per-cell active counts: bucket 0 nact <= 16,
+// 1: <= 32, 2: <= 64.  counts[0..2] and lists[b * ncell + i] (device); the
+// counts are cleared by a fill kernel on `stream` first.
+constexpr int kBuckets = 3;
+constexpr int kBucketBound[kBuckets] = {16, 32, 64};
+hipError_t launch_buckets(const int32_t* nact, int64_t ncell, int naq, int32_t* lists,
+                          int32_t* counts, hipStream_t stream);
+
 // sensitivities (tangent: mode 0, out = df per cell; adjoint: mode 1, out =
 // per-cell gradients [q * ldo + cell], sum = their sum over converged cells)
 int sens_supported(const DevNet& n, const pfchem_options& opt);

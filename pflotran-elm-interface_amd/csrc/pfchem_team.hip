@@ -69,7 +69,9 @@ int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed) {
   for (const auto& sh : kShapes)
     if (sh[0] == L && sh[1] == R) known = true;
   if (!known) return 0;
-  if (n.ncomp > L * R || n.nm > kMaxMnrl) return 0;
+  // with per-cell active counts the size buckets bring every cell within
+  // its launch's team (nact <= L * R)
+  if ((n.ncomp > L * R && !n.variable) || n.nm > kMaxMnrl) return 0;
   if (n.has_somdec && !team_has_somdec(L, R)) return 0;
   if (n.variable && !team_has_variable(L, R)) return 0;
   if (cfg.block_size % 64 || cfg.block_size > 256 || cfg.block_size < 64) return 0;
@@ -78,6 +80,30 @@ int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed) {
       (size_t)cfg.lds_limit)
     return 0;
   return 1;
+}
+
+namespace {
+__global__ void clear_bucket_counts(int32_t* counts) {
+  if (threadIdx.x < kBuckets) counts[threadIdx.x] = 0;
+}
+__global__ void __launch_bounds__(256)
+bucket_cells(const int32_t* nact, int64_t ncell, int naq, int32_t* lists, int32_t* counts) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int a = min(max(nact[c], 1), naq);
+  const int b = a <= kBucketBound[0] ? 0 : (a <= kBucketBound[1] ? 1 : 2);
+  const int pos = atomicAdd(&counts[b], 1);
+  lists[(int64_t)b * ncell + pos] = (int32_t)c;
+}
+}  // namespace
+
+hipError_t launch_buckets(const int32_t* nact, int64_t ncell, int naq, int32_t* lists,
+                          int32_t* counts, hipStream_t stream) {
+  clear_bucket_counts<<<1, 64, 0, stream>>>(counts);
+  const int block = 256;
+  bucket_cells<<<dim3((unsigned)((ncell + block - 1) / block)), dim3(block), 0, stream>>>(
+      nact, ncell, naq, lists, counts);
+  return hipGetLastError();
 }
 
 hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,

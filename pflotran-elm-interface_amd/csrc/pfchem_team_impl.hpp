@@ -526,7 +526,10 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 
   // XCD of this workgroup (workgroups are dispatched round robin over XCDs)
   const int xcd = (int)(blockIdx.x % (unsigned)nxcd);
-  const int64_t chunk = (ncell + nxcd - 1) / nxcd;
+  // cells of this launch: all, or with VAR a size bucket's list
+  const bool listed = VAR && s.cell_list != nullptr;
+  const int64_t nwork = listed ? (int64_t)gld(s.list_count) : ncell;
+  const int64_t chunk = (nwork + nxcd - 1) / nxcd;
 
   // team-uniform cell state
   int64_t cell = -1;
@@ -569,7 +572,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       while (taken < cnt && steal < nxcd) {
         const int xr = (xcd + steal) % nxcd;
         const long long lo = (long long)xr * chunk;
-        const long long hi = min((long long)ncell, lo + chunk);
+        const long long hi = min((long long)nwork, lo + chunk);
         long long base = 0;
         if (lane == 0) base = (long long)atomicAdd(cursors + 16 * xr, (unsigned long long)batch);
         base = __shfl(base, 0, 64);
@@ -586,16 +589,17 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         qnext += take;
         taken += take;
       }
-      if (needy && got < 0) got = ncell;  // nothing left anywhere
+      if (needy && got < 0) got = nwork;  // nothing left anywhere
     }
     got = __shfl(got, team_in_wave * L, 64);
     if (cell < 0 && !exhausted && got >= 0) {
-      if (got >= ncell) {
+      const long long gc = (got < nwork && listed) ? (long long)gld(s.cell_list + got) : got;
+      if (got >= nwork) {
         exhausted = true;
-      } else if (rescue_only && gld(s.status + (got)) != kEscalate) {
+      } else if (rescue_only && gld(s.status + (gc)) != kEscalate) {
         // FP64 rescue pass: only cells the FP32 pass handed over
       } else {
-        cell = got;
+        cell = gc;
         const int64_t c = cell;
         if constexpr (VAR) {
           naq_var = min(max(gld(s.ncomp_active + (c)), 1), n.naq);
