@@ -209,7 +209,8 @@ def main():
     roofline = {"bound": "valu", "achieved": achieved, "peak": MI355X_FP64_VALU_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_VALU_TFLOPS,
                 "traffic": traffic,
-                "kernel": "team_step_kernel<16,float>",
+                "kernel": "team_step_kernel<16,1,float> (FP32 pass; the FP64 rescue pass "
+                          "over escalated cells is inside the same step)",
                 "algorithmic_flop_per_launch": flop_alg,
                 "algorithmic_bytes_per_launch": bytes_alg,
                 "hbm_gbs_algorithmic": bytes_alg / (k_ms * 1e-3) / 1e9,
