@@ -122,6 +122,34 @@ def main():
     value = args.steps * n / (mix_ms * 1e-3)
     log("[bench] batched_mixed %.2f ms/step  %.3e cells/s (wall %.3f s)" % (ms_per_step, value, wall))
 
+    # --- GIRT reaction block of one iterate (pfchem_residual_jacobian) ---
+    girt = None
+    naq = net.naqcomp
+    gst_ = copies(1)[0]
+    fixed = gst_.t["total"][:naq].clone()
+    ctx.residual(gst_, fixed, None, wl.dt)
+    torch.cuda.synchronize()
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
+    reps = 5
+    g0.record()
+    for _ in range(reps):
+        ctx.residual(gst_, fixed, None, wl.dt)
+    g1.record()
+    torch.cuda.synchronize()
+    g_ms = g0.elapsed_time(g1) / reps
+    ncomp = naq + net.nimmobile
+    # algorithmic bytes per cell: state read (free molalities, activity
+    # coefficients, fixed totals, mineral state, 5 cell scalars) and written
+    # (totals, secondary molalities, rates, residual, Jacobian block)
+    g_bytes = 8 * (3 * naq + net.neqcplx + 2 * net.nkinmnrl + 5 +
+                   naq + net.neqcplx + net.nkinmnrl + ncomp + ncomp * ncomp)
+    girt = {"kernel": "residual_kernel (pfchem_residual_jacobian)", "ms": g_ms,
+            "cells_per_s": n / (g_ms * 1e-3), "bytes_per_cell": g_bytes,
+            "hbm_gbs_algorithmic": g_bytes * n / (g_ms * 1e-3) / 1e9}
+    log("[bench] girt residual+jacobian %.2f ms  %.3e cells/s" % (g_ms, n / (g_ms * 1e-3)))
+    del gst_, fixed
+
     got = mix_state.to_host()
     status = got["status"][0, :n]
     iters_gpu = got["num_iterations"][0, :n]
@@ -234,6 +262,7 @@ def main():
         "plain_fp64_cells_per_s": plain_cps,
         "batched_fp64_cells_per_s": b64_cps,
         "speedup_vs_plain_fp64": value / plain_cps,
+        "girt_residual_jacobian": girt,
         "failed_cells": int((status != 0).sum()),
         "newton_iterations_mean": float(iters_gpu.mean()),
     }

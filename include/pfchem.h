@@ -20,9 +20,13 @@
  *                        called per cell by RTUpdateActivityCoefficients
  *                        (reactive_transport.F90:3441-3492) at the start of
  *                        an OSRT step (pm_osrt.F90:148-151).
- *   pfchem_residual    <- RTAccumulation + RReaction (reaction.F90:5710,
- *                        4059): the per-cell residual of one RReact
- *                        iterate, FP64 (used by GIRT callers and checks).
+ *   pfchem_residual / pfchem_residual_jacobian <- the per-cell reaction
+ *                        block of the global implicit residual and Jacobian:
+ *                        RTAccumulation + RReaction in RTResidualNonFlux
+ *                        (reactive_transport.F90:2407, 2627) and
+ *                        RTAccumulationDerivative + RReactionDerivative in
+ *                        RTJacobianNonFlux (:3177, 3288), in the RReact form
+ *                        of one Newton iterate (reaction.F90:3896-3916), FP64.
  *   pfchem_tangent / pfchem_adjoint: no reference counterpart (the
  *                        reference has no sensitivities); they differentiate
  *                        the converged RReact system (reaction.F90:3853-4049).
@@ -311,13 +315,32 @@ int pfchem_update_activity(pfchem_ctx* ctx, int64_t ncell,
                            const pfchem_state* state, void* stream);
 
 /* FP64 residual of the RReact system at the state's current free
-   molalities (reaction.F90:3899-3916): res[i*ld+cell] =
-   (Acc(x) - Acc(total^k,imm^k))/dt + reaction terms, with the fixed
-   accumulation taken from `fixed_total` / `fixed_immobile` (SoA).
-   Device pointers. */
+   molalities and activity coefficients (reaction.F90:3896-3916):
+   res[i*ld+cell] = (Acc(x) - Acc(total^k,imm^k))/dt + reaction terms, with
+   the fixed accumulation taken from the totals `fixed_total` [naqcomp][ld]
+   and `fixed_immobile` [nimmobile][ld] (SoA; NULL when the count is 0).
+   Rows i < naqcomp are the aqueous primaries, then the immobile species.
+   As RTAuxVarCompute does, the call rewrites state->total and
+   state->sec_molal from the free molalities, and state->mnrl_rate.  Dry
+   cells (saturation < 1e-40) give res = 0.  Networks with the SomDec sandbox
+   are refused (PFCHEM_ERR_UNSUPPORTED).  Device pointers; launches on
+   `stream` without synchronizing. */
 int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                     const double* fixed_total, const double* fixed_immobile,
                     double dt, double* res, void* stream);
+
+/* pfchem_residual plus the Jacobian block of the same iterate:
+   jac[(i*ncomp + j)*ld + cell] = d res_i / d x_j (ncomp = naqcomp +
+   nimmobile; aqueous columns per mol/kg water of free molality, immobile
+   columns per mol/m^3 bulk; identity for a dry cell), the analytic
+   RTAccumulationDerivative + RReaction derivative that RTJacobianNonFlux
+   adds into the cell's diagonal block (reactive_transport.F90:3177, 3288).
+   jac may be NULL.  Replaces the per-cell loop bodies of RTResidualNonFlux
+   (reactive_transport.F90:2393-2430, 2599-2642) and RTJacobianNonFlux
+   (:3172-3183, 3270-3300) for a batch of cells. */
+int pfchem_residual_jacobian(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                             const double* fixed_total, const double* fixed_immobile,
+                             double dt, double* res, double* jac, void* stream);
 
 /* Launch configuration of the batched kernels, set by the autotuner: a
    team of lanes_per_cell lanes owns one cell (64 / lanes_per_cell cells per

@@ -680,12 +680,37 @@ int pfchem_update_activity(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* s
   return PFCHEM_OK;
 }
 
+int pfchem_residual_jacobian(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                             const double* fixed_total, const double* fixed_immobile, double dt,
+                             double* res, double* jac, void* stream) {
+  if (!ctx) return PFCHEM_ERR_INVALID;
+  if (ncell < 0) return fail(ctx, PFCHEM_ERR_INVALID, "ncell < 0");
+  if (!(dt > 0.0)) return fail(ctx, PFCHEM_ERR_INVALID, "dt must be > 0");
+  if (ctx->dnet.has_somdec)
+    // the one-cell-per-thread SomDec code faulted on MI355X in the plain
+    // step kernel without a cause found; it is not reused here
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "the residual does not evaluate the SomDec sandbox");
+  if (!res) return fail(ctx, PFCHEM_ERR_INVALID, "null res");
+  if (ctx->dnet.naq > 0 && !fixed_total) return fail(ctx, PFCHEM_ERR_INVALID, "null fixed_total");
+  if (ctx->dnet.nimm > 0 && !fixed_immobile)
+    return fail(ctx, PFCHEM_ERR_INVALID, "null fixed_immobile");
+  DevState d;
+  int rc = to_dev_state(ctx, state, ncell, &d);
+  if (rc) return rc;
+  if (ncell == 0) return PFCHEM_OK;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess)
+    e = launch_residual(ctx->dnet, ctx->dnet_dev, d, ncell, dt, fixed_total, fixed_immobile, res,
+                        jac, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "residual launch");
+  return PFCHEM_OK;
+}
+
 int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                     const double* fixed_total, const double* fixed_immobile, double dt,
                     double* res, void* stream) {
-  (void)ncell; (void)state; (void)fixed_total; (void)fixed_immobile; (void)dt; (void)res;
-  (void)stream;
-  return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "pfchem_residual not available yet");
+  return pfchem_residual_jacobian(ctx, ncell, state, fixed_total, fixed_immobile, dt, res,
+                                  nullptr, stream);
 }
 
 int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg) {

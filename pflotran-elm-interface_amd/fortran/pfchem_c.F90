@@ -228,6 +228,21 @@ module pfchem_c
       integer(c_int), value :: variant
     end function pfchem_step_host
 
+    ! GIRT reaction block of one iterate: res(i*ld+cell), jac((i*ncomp+j)*ld+cell)
+    integer(c_int) function pfchem_residual_jacobian(ctx, ncell, state, fixed_total, &
+        fixed_immobile, dt, res, jac, stream) bind(C, name="pfchem_residual_jacobian")
+      import :: c_int, c_int64_t, c_double, c_ptr, pfchem_state
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ncell
+      type(pfchem_state), intent(in) :: state
+      type(c_ptr), value :: fixed_total
+      type(c_ptr), value :: fixed_immobile
+      real(c_double), value :: dt
+      type(c_ptr), value :: res
+      type(c_ptr), value :: jac
+      type(c_ptr), value :: stream
+    end function pfchem_residual_jacobian
+
     integer(c_int) function pfchem_set_launch(ctx, variant, cfg) bind(C, name="pfchem_set_launch")
       import :: c_int, c_ptr, pfchem_launch
       type(c_ptr), value :: ctx

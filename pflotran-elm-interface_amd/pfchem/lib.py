@@ -48,6 +48,7 @@ def load(path: str = LIB_PATH):
     L.pfchem_step_host.argtypes = [vp, C.c_int64, vp, C.c_double, C.c_int]
     L.pfchem_update_activity.argtypes = [vp, C.c_int64, vp, vp]
     L.pfchem_residual.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp]
+    L.pfchem_residual_jacobian.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, vp]
     L.pfchem_set_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_get_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_tangent.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, vp, C.c_int, vp, vp, vp,
@@ -163,6 +164,28 @@ class Context:
         sp = (stream or torch.cuda.current_stream()).cuda_stream
         self._check(self.L.pfchem_update_activity(self.h, st.ncell, C.byref(s), sp),
                     "pfchem_update_activity")
+
+    def residual(self, st: DeviceState, fixed_total, fixed_immobile, dt: float,
+                 jacobian: bool = True, stream=None):
+        """pfchem_residual_jacobian: the GIRT reaction block of one iterate at
+        the state's free molalities.  fixed_total / fixed_immobile: device
+        tensors (naqcomp, ld) / (nimmobile, ld) or None.  Returns (res
+        (ncomp, ld), jac (ncomp * ncomp, ld) or None) as device tensors; the
+        call also rewrites total, sec_molal and mnrl_rate in `st`."""
+        import torch
+        s = st.to_abi()
+        sp = (stream or torch.cuda.current_stream()).cuda_stream
+        ncomp = self.net.naqcomp + self.net.nimmobile
+        res = torch.zeros((ncomp, st.ld), dtype=torch.float64, device="cuda")
+        jac = (torch.zeros((ncomp * ncomp, st.ld), dtype=torch.float64, device="cuda")
+               if jacobian else None)
+
+        def ptr(t):
+            return None if t is None or t.numel() == 0 else t.data_ptr()
+        self._check(self.L.pfchem_residual_jacobian(self.h, st.ncell, C.byref(s), ptr(fixed_total),
+                                                    ptr(fixed_immobile), dt, res.data_ptr(),
+                                                    ptr(jac), sp), "pfchem_residual_jacobian")
+        return res, jac
 
     def set_launch(self, variant, lanes_per_cell, block_size=256, blocks_per_cu=4,
                    rows_per_lane=1):

@@ -1,0 +1,166 @@
+"""GPU reaction block of the global implicit residual and Jacobian
+(pfchem_residual_jacobian: RTResidualNonFlux / RTJacobianNonFlux per cell,
+reactive_transport.F90:2393-2430, 2599-2642, 3172-3183, 3270-3300, in the
+RReact form of reaction.F90:3896-3916) against the FP64 oracle's
+orc_total + orc_rreact_system on the same cells.
+
+Inputs: the workload's free molalities, and fixed (previous-time) totals
+drawn around the cells' current totals, so that the accumulation term does
+not cancel to zero; a few cells are dry.
+
+Bars: residual relative error <= 1e-9, with a floor of 1e-12 of the
+accumulation rate psv*total/dt of that row (the residual is a difference of
+two accumulations); Jacobian entries <= 1e-11 of the row's largest entry;
+totals and secondary molalities written back <= 1e-13 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from pfchem import abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+DT = 3600.0
+
+
+def _case(which, ncell, seed=7):
+    if which == "calcite":
+        wl = workloads.calcite_workload(ncell)
+    else:
+        wl = workloads.random_network_workload(ncell=ncell)
+    st = wl.state.copy()
+    rng = np.random.default_rng(seed)
+    n = st.ncell
+    st["saturation"][0, rng.choice(n, size=max(1, n // 100), replace=False)] = 0.0
+    naq = wl.net.naqcomp
+    fixed = st["total"][:naq].copy()
+    fixed[:, :n] *= 1.0 + 1e-3 * rng.standard_normal(fixed[:, :n].shape)
+    return wl, st, fixed
+
+
+def _oracle(wl, st, fixed, cells):
+    net = wl.net
+    ncomp = net.naqcomp + net.nimmobile
+    res = np.zeros((ncomp, cells.size))
+    jac = np.zeros((ncomp, ncomp, cells.size))
+    tot = np.zeros((net.naqcomp, cells.size))
+    sec = np.zeros((net.neqcplx, cells.size))
+    acc = np.zeros((ncomp, cells.size))
+    cx = O.CellCtx(net, wl.opt)
+    for q, c in enumerate(cells):
+        cc = cx.gather(st, int(c))
+        cc.total()
+        por, sat = st["porosity"][0, c], st["saturation"][0, c]
+        vol = st["volume"][0, c]
+        fa = np.zeros(ncomp)
+        if sat >= 1e-40:
+            psv = por * sat * 1000.0 * vol
+            fa[:net.naqcomp] = psv * fixed[:, c]
+            acc[:net.naqcomp, q] = np.abs(psv * cc.get("total", net.naqcomp)) / DT
+        r, J = cc.rreact_system(fa, DT)
+        res[:, q] = r
+        jac[:, :, q] = J
+        tot[:, q] = cc.get("total", net.naqcomp)
+        sec[:, q] = cc.get("sec_molal", net.neqcplx)
+    return res, jac, tot, sec, acc
+
+
+@pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048)])
+def test_residual_jacobian_matches_oracle(which, ncell):
+    import torch
+    from pfchem.lib import Context, DeviceState
+    wl, st, fixed = _case(which, ncell)
+    print("case residual", which, ncell, flush=True)
+    n = st.ncell
+    cells = np.arange(n) if n <= 1000 else np.unique(np.r_[np.arange(0, n, 3), n - 1])
+    r_ref, j_ref, t_ref, s_ref, acc = _oracle(wl, st, fixed, cells)
+    ctx = Context(wl.net, wl.opt)
+    ds = DeviceState(st)
+    ft = torch.from_numpy(np.ascontiguousarray(fixed)).cuda()
+    res, jac = ctx.residual(ds, ft, None, DT)
+    torch.cuda.synchronize()
+    res = res.cpu().numpy()[:, cells]
+    ncomp = wl.net.naqcomp + wl.net.nimmobile
+    jac = jac.cpu().numpy().reshape(ncomp, ncomp, -1)[:, :, cells]
+    got = ds.to_host()
+    d = np.abs(res - r_ref)
+    floor = 1e-12 * acc
+    rel = np.where(d == 0, 0.0, d / np.maximum(np.abs(r_ref), np.maximum(floor, 1e-300)))
+    print(which, "residual max rel err %.3e" % rel.max(), flush=True)
+    assert rel.max() <= 1e-9
+    scale = np.abs(j_ref).max(axis=1, keepdims=True)
+    scale = np.where(scale > 0, scale, 1.0)
+    je = (np.abs(jac - j_ref) / scale).max()
+    print(which, "jacobian max err / row scale %.3e" % je, flush=True)
+    assert je <= 1e-11
+    te = (np.abs(got["total"][:, cells] - t_ref) / np.abs(t_ref)).max()
+    assert te <= 1e-13
+    if wl.net.neqcplx:
+        se = (np.abs(got["sec_molal"][:, cells] - s_ref) / np.abs(s_ref)).max()
+        print(which, "totals %.3e sec %.3e" % (te, se), flush=True)
+        assert se <= 1e-13
+    # dry cells: zero residual, identity Jacobian
+    dry = np.where(st["saturation"][0, cells] < 1e-40)[0]
+    assert dry.size > 0
+    assert np.all(res[:, dry] == 0.0)
+    eye = np.eye(ncomp)[:, :, None]
+    assert np.all(jac[:, :, dry] == eye)
+
+
+def test_residual_vanishes_at_a_converged_step():
+    """pfchem_step's FP64 answer, fed back with the step's initial totals as
+    the fixed accumulation, leaves a residual at the Newton tolerance for the
+    cells that took one sub-step (the two kernels share nothing but the
+    network): largest entry <= 1e-10 of the cell's largest accumulation
+    rate."""
+    import torch
+    from pfchem.lib import Context, DeviceState
+    wl = workloads.random_network_workload(ncell=4096)
+    print("case residual at converged step", flush=True)
+    n = wl.state.ncell
+    naq = wl.net.naqcomp
+    ctx = Context(wl.net, wl.opt)
+    ds = DeviceState(wl.state)
+    # RStep solves against totals clamped at 1e-40 (reaction.F90:3617-3631)
+    fixed = ds.t["total"][:naq].clone().clamp(min=1e-40)
+    ctx.step(ds, wl.dt, abi.KERNEL_PLAIN_FP64)
+    res, _ = ctx.residual(ds, fixed, None, wl.dt, jacobian=False)
+    torch.cuda.synchronize()
+    got = ds.to_host()
+    one = (got["num_sub_steps"][0, :n] == 1) & (got["status"][0, :n] == 0)
+    # a mineral that dissolved away in the step has no rate at the new state
+    gone = np.any((got["mnrl_volfrac"][:, :n] == 0.0) & (wl.state["mnrl_volfrac"][:, :n] > 0.0),
+                  axis=0)
+    one &= ~gone
+    assert one.mean() > 0.9
+    psv = (wl.state["porosity"][0, :n] * wl.state["saturation"][0, :n] * 1000.0 *
+           wl.state["volume"][0, :n])
+    # per cell: largest residual entry over the largest accumulation rate
+    # (single totals can cancel to ~0, e.g. the proton's)
+    acc = (psv[None, :] * np.abs(got["total"][:naq, :n]) / wl.dt).max(axis=0)
+    r = np.abs(res.cpu().numpy()[:naq, :n]).max(axis=0)
+    rel = (r / np.maximum(acc, 1e-300))[one]
+    print("max |res| / accumulation rate %.3e" % rel.max(), flush=True)
+    assert rel.max() <= 1e-10
+
+
+def test_residual_argument_checks():
+    import torch
+    from pfchem.lib import Context, DeviceState, PfchemError
+    wl = workloads.calcite_workload(64)
+    ctx = Context(wl.net, wl.opt)
+    ds = DeviceState(wl.state)
+    with pytest.raises(PfchemError):
+        ctx.residual(ds, None, None, 1.0)
+    ft = ds.t["total"][:wl.net.naqcomp].clone()
+    with pytest.raises(PfchemError):
+        ctx.residual(ds, ft, None, 0.0)
+    sd = workloads.somdec_workload(ncell=64)
+    ctx2 = Context(sd.net, sd.opt)
+    ds2 = DeviceState(sd.state)
+    with pytest.raises(PfchemError) as e:
+        ctx2.residual(ds2, ds2.t["total"][:sd.net.naqcomp].clone(),
+                      ds2.t["immobile"].clone(), 1800.0)
+    assert e.value.code == abi.PFCHEM_ERR_UNSUPPORTED
+    torch.cuda.synchronize()
