@@ -144,7 +144,7 @@ def main():
     # (totals, secondary molalities, rates, residual, Jacobian block)
     g_bytes = 8 * (3 * naq + net.neqcplx + 2 * net.nkinmnrl + 5 +
                    naq + net.neqcplx + net.nkinmnrl + ncomp + ncomp * ncomp)
-    girt = {"kernel": "residual_kernel (pfchem_residual_jacobian)", "ms": g_ms,
+    girt = {"kernel": "girt_team_kernel (pfchem_residual_jacobian)", "ms": g_ms,
             "cells_per_s": n / (g_ms * 1e-3), "bytes_per_cell": g_bytes,
             "hbm_gbs_algorithmic": g_bytes * n / (g_ms * 1e-3) / 1e9}
     log("[bench] girt residual+jacobian %.2f ms  %.3e cells/s" % (g_ms, n / (g_ms * 1e-3)))

@@ -699,9 +699,16 @@ int pfchem_residual_jacobian(pfchem_ctx* ctx, int64_t ncell, const pfchem_state*
   if (rc) return rc;
   if (ncell == 0) return PFCHEM_OK;
   hipError_t e = hipSetDevice(ctx->device);
-  if (e == hipSuccess)
-    e = launch_residual(ctx->dnet, ctx->dnet_dev, d, ncell, dt, fixed_total, fixed_immobile, res,
-                        jac, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  hipStream_t s = (hipStream_t)stream;
+  DevState* slot = ctx->dstate_ring + ctx->ring_next;
+  DevState* hslot = ctx->hstate_ring + ctx->ring_next;
+  ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
+  *hslot = d;
+  e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+  e = launch_girt_team(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, fixed_total, fixed_immobile, res,
+                       jac, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "residual launch");
   return PFCHEM_OK;
 }

@@ -21,11 +21,14 @@ hipError_t launch_plain_step(const DevNet& n, const DevNet* dn, const DevState& 
                              double dt, hipStream_t stream);
 hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                                hipStream_t stream);
-// GIRT reaction block of one iterate (one cell per thread, FP64): res[i*ld+c]
-// and, if jac != nullptr, jac[(i*ncomp+j)*ld+c]
-hipError_t launch_residual(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
-                           double dt, const double* fixed_total, const double* fixed_imm,
-                           double* res, double* jac, hipStream_t stream);
+// GIRT reaction block of one iterate, FP64: res[i*ld+c] and, if jac !=
+// nullptr, jac[(i*ncomp+j)*ld+c]; teams of L >= ncomp lanes (one row per
+// lane, 256 / L cells per workgroup); ds: device copy of the state
+// descriptor.  girt_team_lanes returns L (0: ncomp > 64)
+int girt_team_lanes(const DevNet& n);
+hipError_t launch_girt_team(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
+                            double dt, const double* fixed_total, const double* fixed_imm,
+                            double* res, double* jac, hipStream_t stream);
 
 // batched (team) kernel: `cursors` holds one 64-bit work cursor per XCD
 // (128-byte stride) that the launcher zeroes with a fill kernel (no memset

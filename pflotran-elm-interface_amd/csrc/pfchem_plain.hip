@@ -502,102 +502,6 @@ activity_kernel(const DevNet* np, DevState s, int64_t ncell) {
   if (s.ln_act_h2o && n.act_h2o) s.ln_act_h2o[c] = w.ln_act_h2o;
 }
 
-
-// The per-cell reaction block of the global implicit (GIRT) residual and
-// Jacobian: RTResidualNonFlux / RTJacobianNonFlux (reactive_transport.F90:
-// 2393-2430, 2599-2642, 3172-3183, 3270-3300) in the RReact form of one
-// Newton iterate (reaction.F90:3896-3916).  At the state's free molalities
-// and activity coefficients, RTotal recomputes totals and secondary
-// molalities (written back, as RTAuxVarCompute updates the auxvar), then
-//   res_i  = (Acc_i(x) - Acc_i(total^k, imm^k)) / dt + R_i(x)
-//   jac_ij = d res_i / d x_j      (RTAccumulationDerivative + the analytic
-//            RReaction derivative; aqueous columns per mol/kg water, immobile
-//            columns per mol/m^3 bulk)
-// and the kinetic mineral rates are written to mnrl_rate.  Rows and columns
-// beyond a cell's active count (options.variable_ncomp) are zero.
-template <int MAXN>
-__global__ void __launch_bounds__(128)
-residual_kernel(const DevNet* np, DevState s, int64_t ncell, double dt,
-                const double* __restrict__ fixed_total, const double* __restrict__ fixed_imm,
-                double* __restrict__ res, double* __restrict__ jac) {
-  const DevNet& n = *np;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  const int64_t ld = s.ld;
-  const int naq = n.variable ? min(max(s.ncomp_active[c], 1), n.naq) : n.naq;
-  const int nimm = n.nimm, N = naq + nimm, NF = n.ncomp;
-  PlainCell<MAXN> w;
-  w.naq = naq;
-  w.N = N;
-  for (int i = 0; i < naq; ++i) {
-    w.x[i] = s.pri_molal[i * ld + c];
-    w.gp[i] = s.pri_act_coef[i * ld + c];
-  }
-  for (int i = 0; i < nimm; ++i) {
-    w.imm[i] = s.immobile[i * ld + c];
-    w.x[naq + i] = w.imm[i];
-  }
-  for (int m = 0; m < n.nm; ++m) {
-    w.vf[m] = s.mnrl_volfrac[m * ld + c];
-    w.logKm[m] = n.mnrl_logK[m];
-    w.rate[m] = 0.0;
-  }
-  w.ln_act_h2o = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
-  w.por = s.porosity[c];
-  w.sat = s.saturation[c];
-  w.den = s.den_kg[c];
-  w.vol = s.volume[c];
-  w.temp = s.temperature ? s.temperature[c] : 25.0;
-  if (!n.isothermal && n.has_mnrl_logKcoef)
-    for (int m = 0; m < n.nm; ++m) w.logKm[m] = interp_logk(n.mnrl_logKcoef + 5 * m, w.temp);
-  // RTotal at the current free molalities (dtotal into the aq block of J)
-  plain_total(n, s, c, w, 1);
-  for (int i = 0; i < naq; ++i) s.total[i * ld + c] = w.total[i];
-  // RTAccumulation / RTAccumulationDerivative against the fixed accumulation
-  const bool wet = !(w.sat < kRtMinSaturation);
-  const double psv = w.por * w.sat * 1000.0 * w.vol;
-  for (int i = 0; i < N; ++i) w.res[i] = 0.0;
-  if (wet) {
-    const double psvd = psv / dt;
-    for (int i = 0; i < naq; ++i) {
-      w.res[i] = (psv * w.total[i] - psv * fixed_total[i * ld + c]) / dt;
-      for (int j = 0; j < naq; ++j) w.J[i * N + j] *= psvd;
-      for (int j = naq; j < N; ++j) w.J[i * N + j] = 0.0;
-    }
-    for (int i = 0; i < nimm; ++i) {
-      const int d = naq + i;
-      w.res[d] = (w.imm[i] * w.vol - fixed_imm[i * ld + c] * w.vol) / dt;
-      for (int j = 0; j < N; ++j) w.J[d * N + j] = 0.0;
-      w.J[d * N + d] = w.vol / dt;
-    }
-    // RReaction (reaction.F90:4059-4130)
-    if (n.nm > 0) plain_minerals(n, s, c, w, w.res, w.J, 1);
-    for (int r = 0; r < n.ndecay; ++r) {
-      const int ic = n.decay_id[r];
-      const double kc = n.decay_k[r] * w.vol;
-      const int d = naq + ic;
-      w.res[d] += kc * w.imm[ic];
-      w.J[d * N + d] += kc;
-    }
-  } else {
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) w.J[i * N + j] = (i == j) ? 1.0 : 0.0;
-  }
-  if (s.mnrl_rate)
-    for (int m = 0; m < n.nm; ++m)
-      if (plain_mnrl_on(n, w, m)) s.mnrl_rate[m * ld + c] = w.rate[m];
-  // rows / columns in the network's numbering: aqueous 0..naq-1, immobile
-  // from n.naq on
-  auto gid = [&](int i) { return i < naq ? i : n.naq + (i - naq); };
-  for (int i = 0; i < NF; ++i) res[i * ld + c] = 0.0;
-  for (int i = 0; i < N; ++i) res[gid(i) * ld + c] = w.res[i];
-  if (jac) {
-    for (int q = 0; q < NF * NF; ++q) jac[q * ld + c] = 0.0;
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) jac[((int64_t)gid(i) * NF + gid(j)) * ld + c] = w.J[i * N + j];
-  }
-}
-
 }  // namespace
 
 int plain_max_ncomp() { return 64; }
@@ -630,29 +534,6 @@ hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState
     activity_kernel<16><<<grid, block, 0, stream>>>(dn, s, ncell);
   else
     activity_kernel<64><<<grid, block, 0, stream>>>(dn, s, ncell);
-  return hipGetLastError();
-}
-
-}  // namespace pfchem
-
-namespace pfchem {
-
-hipError_t launch_residual(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
-                           double dt, const double* fixed_total, const double* fixed_imm,
-                           double* res, double* jac, hipStream_t stream) {
-  const int block = 128;
-  const dim3 grid((unsigned)((ncell + block - 1) / block));
-  if (n.has_somdec) return hipErrorInvalidValue;  // refused by pfchem_residual
-  if (n.ncomp <= 4)
-    residual_kernel<4><<<grid, block, 0, stream>>>(dn, s, ncell, dt, fixed_total, fixed_imm, res, jac);
-  else if (n.ncomp <= 8)
-    residual_kernel<8><<<grid, block, 0, stream>>>(dn, s, ncell, dt, fixed_total, fixed_imm, res, jac);
-  else if (n.ncomp <= 16)
-    residual_kernel<16><<<grid, block, 0, stream>>>(dn, s, ncell, dt, fixed_total, fixed_imm, res, jac);
-  else if (n.ncomp <= 32)
-    residual_kernel<32><<<grid, block, 0, stream>>>(dn, s, ncell, dt, fixed_total, fixed_imm, res, jac);
-  else
-    residual_kernel<64><<<grid, block, 0, stream>>>(dn, s, ncell, dt, fixed_total, fixed_imm, res, jac);
   return hipGetLastError();
 }
 
