@@ -322,9 +322,9 @@ int pfchem_update_activity(pfchem_ctx* ctx, int64_t ncell,
    Rows i < naqcomp are the aqueous primaries, then the immobile species.
    As RTAuxVarCompute does, the call rewrites state->total and
    state->sec_molal from the free molalities, and state->mnrl_rate.  Dry
-   cells (saturation < 1e-40) give res = 0.  Networks with the SomDec sandbox
-   are refused (PFCHEM_ERR_UNSUPPORTED).  Device pointers; launches on
-   `stream` without synchronizing. */
+   cells (saturation < 1e-40) give res = 0.  The SomDec sandbox's terms are
+   included (per-cell stoichiometry from the Setup values).  Device
+   pointers; launches on `stream` without synchronizing. */
 int pfchem_residual(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                     const double* fixed_total, const double* fixed_immobile,
                     double dt, double* res, void* stream);

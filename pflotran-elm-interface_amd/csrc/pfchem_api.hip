@@ -686,10 +686,6 @@ int pfchem_residual_jacobian(pfchem_ctx* ctx, int64_t ncell, const pfchem_state*
   if (!ctx) return PFCHEM_ERR_INVALID;
   if (ncell < 0) return fail(ctx, PFCHEM_ERR_INVALID, "ncell < 0");
   if (!(dt > 0.0)) return fail(ctx, PFCHEM_ERR_INVALID, "dt must be > 0");
-  if (ctx->dnet.has_somdec)
-    // the one-cell-per-thread SomDec code faulted on MI355X in the plain
-    // step kernel without a cause found; it is not reused here
-    return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "the residual does not evaluate the SomDec sandbox");
   if (!res) return fail(ctx, PFCHEM_ERR_INVALID, "null res");
   if (ctx->dnet.naq > 0 && !fixed_total) return fail(ctx, PFCHEM_ERR_INVALID, "null fixed_total");
   if (ctx->dnet.nimm > 0 && !fixed_immobile)

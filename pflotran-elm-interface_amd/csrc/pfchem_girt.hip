@@ -8,7 +8,8 @@
 // RReact form of one Newton iterate (reaction.F90:3896-3916): RTotalAqueous
 // (:4665-4759), RTAccumulation(Derivative) (:5710-5848), RReaction (:4059)
 // -> RKineticMineral (reaction_mineral.F90:647-1078) and RImmobileDecay
-// (reaction_immobile.F90:244-294).
+// (reaction_immobile.F90:244-294), and the SomDec sandbox (SomDecReact,
+// reaction_sandbox_somdec.F90:1504-1910, through pfchem_somdec.hpp).
 //
 // Per cell:  res_i  = (Acc_i(x) - Acc_i(total^k, imm^k)) / dt + R_i(x)
 //            jac_ij = d res_i / d x_j   (linear unknowns: free molality for
@@ -25,19 +26,54 @@
 namespace pfchem {
 namespace {
 
-// LDS of one team (doubles): lnact, 1/x, totals accumulator, reaction
-// residual accumulator, final residual, Jacobian tile NM x (NM + 1)
-__host__ __device__ constexpr int girt_team_doubles(int NM) { return 5 * NM + NM * (NM + 1); }
+// LDS of one team (doubles): lnact, 1/x, x, totals accumulator, reaction
+// residual accumulator, final residual, Jacobian tile NM x (NM + 1), then
+// nsd doubles of per-cell SomDec stoichiometry
+__host__ __device__ constexpr int girt_team_doubles(int NM, int nsd) {
+  return 6 * NM + NM * (NM + 1) + nsd;
+}
+constexpr int kGirtJ = 6;  // offset of the Jacobian tile, in units of NM
 
-template <int L>
+// SomDec accumulator of a GIRT team: species values from the team's LDS,
+// residual terms into the reaction residual, Jacobian terms d res_row /
+// d x_col added as they come (the tile is linear here)
+struct GirtSdEnv {
+  typedef __attribute__((address_space(3))) double lds_t;
+  const lds_t* xv;
+  const lds_t* lna;
+  lds_t* racc;
+  lds_t* Jt;
+  lds_t* st;
+  int naq, nrxn, ld;
+  double dkl, theta, vol, sat, temp, f_t, f_w;
+  bool has_f_t, has_f_w;
+  __device__ double total(int i) const { return xv[i] * dkl; }
+  __device__ double imm(int i) const { return xv[naq + i]; }
+  __device__ double lnact(int i) const { return lna[i]; }
+  __device__ double dtot(int i, int j) const { return i == j ? dkl : 0.0; }
+  __device__ void R(int row, double v) {
+    __hip_atomic_fetch_add(&racc[row], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ void J(int row, int col, double v) {
+    __hip_atomic_fetch_add(&Jt[row * ld + col], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  __device__ lds_t& unc(int r) { return st[r]; }
+  __device__ lds_t& mcs(int r) { return st[nrxn + r]; }
+  __device__ lds_t& mns(int r) { return st[2 * nrxn + r]; }
+  __device__ lds_t& dnc(int r, int j) { return st[3 * nrxn + r * kSdDown + j]; }
+  static __device__ lds_t* lds(double* p) { return (lds_t*)p; }
+};
+
+template <int L, bool SD>
 __global__ void __launch_bounds__(256)
 girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                  const double* __restrict__ fixed_total, const double* __restrict__ fixed_imm,
                  double* __restrict__ res, double* __restrict__ jac) {
   constexpr int NM = L;
-  constexpr int TD = girt_team_doubles(NM);
   const DevNet& n = *np;
   const DevState& s = *sp;
+  const int nsd = SD ? n.sd.nrxn * (3 + kSdDown) : 0;
+  const int TD = girt_team_doubles(NM, nsd);
   extern __shared__ __align__(16) char smem[];
   const int teams = (int)(blockDim.x / L);
   const int team = (int)(threadIdx.x / L);
@@ -45,10 +81,12 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   double* tb = (double*)smem + (size_t)team * TD;
   double* lnact = tb;
   double* xinv = tb + NM;
-  double* tot = tb + 2 * NM;
-  double* racc = tb + 3 * NM;
-  double* rfin = tb + 4 * NM;
-  double* J = tb + 5 * NM;
+  double* xv = tb + 2 * NM;
+  double* tot = tb + 3 * NM;
+  double* racc = tb + 4 * NM;
+  double* rfin = tb + 5 * NM;
+  double* J = tb + kGirtJ * NM;
+  double* sdst = J + NM * (NM + 1);
   int* nrow = (int*)((double*)smem + (size_t)teams * TD);  // unknowns per team's cell
   const int64_t c0 = (int64_t)blockIdx.x * teams;
   const int64_t c = c0 + team;
@@ -82,9 +120,21 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       lnact[i] = log(x);
     }
     xinv[i] = 1.0 / x;
+    xv[i] = x;
     tot[i] = 0.0;
     racc[i] = 0.0;
     for (int j = 0; j <= NM; ++j) J[i * (NM + 1) + j] = 0.0;
+    if constexpr (SD) {
+      // per-cell SomDec stoichiometry starts from the Setup values
+      const int nr = n.sd.nrxn;
+      for (int r = t; r < nr; r += L) {
+        sdst[r] = gld(n.sd.upstream_nc + r);
+        sdst[nr + r] = gld(n.sd.mineral_c_stoich + r);
+        sdst[2 * nr + r] = gld(n.sd.mineral_n_stoich + r);
+        for (int j = 0; j < kSdDown; ++j)
+          sdst[3 * nr + r * kSdDown + j] = gld(n.sd.downstream_nc + r * kSdDown + j);
+      }
+    }
     team_sync();
 
     // (B) complexes k = t, t+L, ...: secondary molality, totals, nu nu m
@@ -177,6 +227,21 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       }
       if (s.mnrl_rate) gst(s.mnrl_rate + (m * ld + c), rt);
     }
+    if (SD && wet) {
+      // SomDec sandbox (SomDecReact, reaction_sandbox_somdec.F90:1504-1910):
+      // reactions r = t, t+L, ...; the N2O emission (lane 0) needs the net N
+      // mineralization of all of them
+      GirtSdEnv e{GirtSdEnv::lds(xv), GirtSdEnv::lds(lnact), GirtSdEnv::lds(racc),
+                  GirtSdEnv::lds(J), GirtSdEnv::lds(sdst), naq, n.sd.nrxn, NM + 1, dkl,
+                  por * sat, vol, sat, temp,
+                  s.somdec_f_t ? gld(s.somdec_f_t + c) : 1.0,
+                  s.somdec_f_w ? gld(s.somdec_f_w + c) : 1.0,
+                  s.somdec_f_t != nullptr, s.somdec_f_w != nullptr};
+      double nmin = 0.0;
+      for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, dt, true);
+      nmin = team_sum<L>(nmin);
+      if (t == 0) sd_nemission(n.sd, e, nmin, dt, true);
+    }
     team_sync();
     if (i < N) rfin[i] = wet ? r + racc[i] : 0.0;
   }
@@ -189,14 +254,14 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     const int b = w % teams, q = w / teams;
     if (c0 + b >= ncell) continue;
     const double* tbb = (const double*)smem + (size_t)b * TD;
-    gst(res + ((int64_t)q * ld + c0 + b), q < nrow[b] ? tbb[4 * NM + q] : 0.0);
+    gst(res + ((int64_t)q * ld + c0 + b), q < nrow[b] ? tbb[5 * NM + q] : 0.0);
   }
   if (jac) {
     for (int w = threadIdx.x; w < nent * teams; w += blockDim.x) {
       const int b = w % teams, q = w / teams;
       if (c0 + b >= ncell) continue;
       const int ii = q / NF, jj = q % NF;
-      const double* Jb = (const double*)smem + (size_t)b * TD + 5 * NM;
+      const double* Jb = (const double*)smem + (size_t)b * TD + kGirtJ * NM;
       const int nb = nrow[b];
       gst(jac + ((int64_t)q * ld + c0 + b), (ii < nb && jj < nb) ? Jb[ii * (NM + 1) + jj] : 0.0);
     }
@@ -207,14 +272,20 @@ template <int L>
 hipError_t launch_girt_L(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                          double dt, const double* fixed_total, const double* fixed_imm,
                          double* res, double* jac, hipStream_t stream) {
-  const int team_bytes = 8 * girt_team_doubles(L);
+  const int nsd = n.has_somdec ? n.sd.nrxn * (3 + kSdDown) : 0;
+  const int team_bytes = 8 * girt_team_doubles(L, nsd);
   int teams = 256 / L;
   while (teams > 1 && (size_t)teams * (team_bytes + 4) > 65536) teams /= 2;
   const int block = teams * L;
   const size_t lds = (size_t)teams * team_bytes + 4 * (size_t)teams;
+  if (lds > 65536) return hipErrorInvalidValue;
   const int64_t grid = (ncell + teams - 1) / teams;
-  girt_team_kernel<L><<<dim3((unsigned)grid), dim3(block), lds, stream>>>(
-      dn, ds, ncell, dt, fixed_total, fixed_imm, res, jac);
+  if (n.has_somdec)
+    girt_team_kernel<L, true><<<dim3((unsigned)grid), dim3(block), lds, stream>>>(
+        dn, ds, ncell, dt, fixed_total, fixed_imm, res, jac);
+  else
+    girt_team_kernel<L, false><<<dim3((unsigned)grid), dim3(block), lds, stream>>>(
+        dn, ds, ncell, dt, fixed_total, fixed_imm, res, jac);
   return hipGetLastError();
 }
 

@@ -4,13 +4,16 @@ reactive_transport.F90:2393-2430, 2599-2642, 3172-3183, 3270-3300, in the
 RReact form of reaction.F90:3896-3916) against the FP64 oracle's
 orc_total + orc_rreact_system on the same cells.
 
-Inputs: the workload's free molalities, and fixed (previous-time) totals
-drawn around the cells' current totals, so that the accumulation term does
-not cancel to zero; a few cells are dry.
+Networks: calcite, the random 15-primary network, and the ELM-CN SomDec
+network (5 aqueous + 10 immobile species, the sandbox's reactions and N2O
+emission).  Inputs: the workload's free molalities / immobile
+concentrations, and fixed (previous-time) totals drawn around the cells'
+current ones, so that the accumulation term does not cancel to zero; a few
+cells are dry.
 
 Bars: residual relative error <= 1e-9, with a floor of 1e-12 of the
-accumulation rate psv*total/dt of that row (the residual is a difference of
-two accumulations); Jacobian entries <= 1e-11 of the row's largest entry;
+magnitudes of the terms it sums (accumulation rate psv*total/dt and the
+reaction term of that row: the residual is their difference); Jacobian entries <= 1e-11 of the row's largest entry;
 totals and secondary molalities written back <= 1e-13 relative.
 """
 import numpy as np
@@ -27,6 +30,8 @@ DT = 3600.0
 def _case(which, ncell, seed=7):
     if which == "calcite":
         wl = workloads.calcite_workload(ncell)
+    elif which == "somdec":
+        wl = workloads.somdec_workload(ncell=ncell)
     else:
         wl = workloads.random_network_workload(ncell=ncell)
     st = wl.state.copy()
@@ -36,10 +41,14 @@ def _case(which, ncell, seed=7):
     naq = wl.net.naqcomp
     fixed = st["total"][:naq].copy()
     fixed[:, :n] *= 1.0 + 1e-3 * rng.standard_normal(fixed[:, :n].shape)
-    return wl, st, fixed
+    fimm = None
+    if wl.net.nimmobile:
+        fimm = st["immobile"][:wl.net.nimmobile].copy()
+        fimm[:, :n] *= 1.0 + 1e-3 * rng.standard_normal(fimm[:, :n].shape)
+    return wl, st, fixed, fimm
 
 
-def _oracle(wl, st, fixed, cells):
+def _oracle(wl, st, fixed, fimm, cells, dt):
     net = wl.net
     ncomp = net.naqcomp + net.nimmobile
     res = np.zeros((ncomp, cells.size))
@@ -57,8 +66,13 @@ def _oracle(wl, st, fixed, cells):
         if sat >= 1e-40:
             psv = por * sat * 1000.0 * vol
             fa[:net.naqcomp] = psv * fixed[:, c]
-            acc[:net.naqcomp, q] = np.abs(psv * cc.get("total", net.naqcomp)) / DT
-        r, J = cc.rreact_system(fa, DT)
+            acc[:net.naqcomp, q] = np.abs(psv * cc.get("total", net.naqcomp)) / dt
+            if net.nimmobile:
+                fa[net.naqcomp:] = vol * fimm[:, c]
+                acc[net.naqcomp:, q] = np.abs(vol * cc.get("immobile", net.nimmobile)) / dt
+        r, J = cc.rreact_system(fa, dt)
+        rx, _ = cc.reaction(dt, derivative=False)
+        acc[:, q] += np.abs(rx)
         res[:, q] = r
         jac[:, :, q] = J
         tot[:, q] = cc.get("total", net.naqcomp)
@@ -66,19 +80,22 @@ def _oracle(wl, st, fixed, cells):
     return res, jac, tot, sec, acc
 
 
-@pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048)])
+@pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048), ("somdec", 1024)])
 def test_residual_jacobian_matches_oracle(which, ncell):
     import torch
     from pfchem.lib import Context, DeviceState
-    wl, st, fixed = _case(which, ncell)
+    wl, st, fixed, fimm = _case(which, ncell)
+    dt = wl.dt if which == "somdec" else DT
     print("case residual", which, ncell, flush=True)
     n = st.ncell
-    cells = np.arange(n) if n <= 1000 else np.unique(np.r_[np.arange(0, n, 3), n - 1])
-    r_ref, j_ref, t_ref, s_ref, acc = _oracle(wl, st, fixed, cells)
+    dry_all = np.where(st["saturation"][0, :n] < 1e-40)[0]
+    cells = np.arange(n) if n <= 1000 else np.unique(np.r_[np.arange(0, n, 3), n - 1, dry_all])
+    r_ref, j_ref, t_ref, s_ref, acc = _oracle(wl, st, fixed, fimm, cells, dt)
     ctx = Context(wl.net, wl.opt)
     ds = DeviceState(st)
     ft = torch.from_numpy(np.ascontiguousarray(fixed)).cuda()
-    res, jac = ctx.residual(ds, ft, None, DT)
+    fi = None if fimm is None else torch.from_numpy(np.ascontiguousarray(fimm)).cuda()
+    res, jac = ctx.residual(ds, ft, fi, dt)
     torch.cuda.synchronize()
     res = res.cpu().numpy()[:, cells]
     ncomp = wl.net.naqcomp + wl.net.nimmobile
@@ -160,7 +177,6 @@ def test_residual_argument_checks():
     ctx2 = Context(sd.net, sd.opt)
     ds2 = DeviceState(sd.state)
     with pytest.raises(PfchemError) as e:
-        ctx2.residual(ds2, ds2.t["total"][:sd.net.naqcomp].clone(),
-                      ds2.t["immobile"].clone(), 1800.0)
-    assert e.value.code == abi.PFCHEM_ERR_UNSUPPORTED
+        ctx2.residual(ds2, ds2.t["total"][:sd.net.naqcomp].clone(), None, 1800.0)
+    assert e.value.code == abi.PFCHEM_ERR_INVALID
     torch.cuda.synchronize()
