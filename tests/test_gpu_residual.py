@@ -180,3 +180,67 @@ def test_residual_argument_checks():
         ctx2.residual(ds2, ds2.t["total"][:sd.net.naqcomp].clone(), None, 1800.0)
     assert e.value.code == abi.PFCHEM_ERR_INVALID
     torch.cuda.synchronize()
+
+
+def test_residual_variable_size_cells():
+    """Per-cell active primaries (64-primary network, nact ~ U{8..64}): each
+    cell's block equals the oracle's on the network restricted to its
+    active primaries (variable_util.sub_network); rows and columns beyond
+    nact are zero."""
+    import torch
+    import variable_util as VU
+    from pfchem.lib import Context, DeviceState
+    from pfchem.network import CellState
+    wl = workloads.variable_network_workload(ncell=256)
+    print("case residual variable 256", flush=True)
+    st = wl.state
+    n = st.ncell
+    naq = wl.net.naqcomp
+    rng = np.random.default_rng(11)
+    fixed = st["total"][:naq].copy()
+    fixed[:, :n] *= 1.0 + 1e-3 * rng.standard_normal(fixed[:, :n].shape)
+    ctx = Context(wl.net, wl.opt)
+    ds = DeviceState(st)
+    res, jac = ctx.residual(ds, torch.from_numpy(np.ascontiguousarray(fixed)).cuda(), None, wl.dt)
+    torch.cuda.synchronize()
+    res = res.cpu().numpy()
+    jac = jac.cpu().numpy().reshape(naq, naq, -1)
+    got = ds.to_host()
+    nact_all = np.clip(st["ncomp_active"][0, :n], 1, naq)
+    worst_r = worst_j = 0.0
+    for c in range(0, n, 4):
+        nact = int(nact_all[c])
+        sub, kc, km = VU.sub_network(wl.net, nact)
+        ss = CellState(sub, 1)
+        for name, a in ss.arrays.items():
+            if name not in st.arrays:
+                continue
+            kind = VU.ROWS.get(name)
+            src = st.arrays[name]
+            if kind == "aq":
+                a[:, 0] = src[:nact, c]
+            elif kind == "c":
+                a[:, 0] = src[kc, c]
+            elif kind == "m":
+                a[:, 0] = src[km, c]
+            elif a.shape[0]:
+                a[:, 0] = src[: a.shape[0], c]
+        opt = wl.opt.__class__(**{**wl.opt.__dict__, "variable_ncomp": False})
+        cc = O.CellCtx(sub, opt).gather(ss, 0)
+        cc.total()
+        psv = (st["porosity"][0, c] * st["saturation"][0, c] * 1000.0 * st["volume"][0, c])
+        r_ref, j_ref = cc.rreact_system(psv * fixed[:nact, c], wl.dt)
+        rx, _ = cc.reaction(wl.dt, derivative=False)
+        floor = 1e-12 * (np.abs(psv * cc.get("total", nact)) / wl.dt + np.abs(rx))
+        d = np.abs(res[:nact, c] - r_ref)
+        worst_r = max(worst_r, float((d / np.maximum(np.abs(r_ref), floor)).max()))
+        scale = np.abs(j_ref).max(axis=1, keepdims=True)
+        worst_j = max(worst_j, float((np.abs(jac[:nact, :nact, c] - j_ref) / scale).max()))
+        assert np.all(res[nact:, c] == 0.0)
+        assert np.all(jac[nact:, :, c] == 0.0) and np.all(jac[:, nact:, c] == 0.0)
+        t_ref = cc.get("total", nact)  # totals can cancel: floor at 1e-13 of the largest
+        np.testing.assert_allclose(got["total"][:nact, c], t_ref, rtol=1e-13,
+                                   atol=1e-13 * np.abs(t_ref).max())
+    print("variable: residual max rel err %.3e, jacobian %.3e" % (worst_r, worst_j), flush=True)
+    assert worst_r <= 1e-9
+    assert worst_j <= 1e-11
