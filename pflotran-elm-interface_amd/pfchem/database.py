@@ -25,9 +25,18 @@ or, for isothermal runs, from the tabulated value at that temperature
 (reaction_database.F90:1775-1781); mineral molar volumes are converted
 from cm^3/mol to m^3/mol (reaction_mineral.F90:459).
 
-A secondary species whose reaction names a primary outside the chosen set
-is not re-expressed (no basis swapping): the chosen set must be a basis of
-the database reactions used.
+The chosen primaries need not be the database's primaries (basis
+swapping, BasisInit :1168-1448): every chosen species that has a database
+reaction (a chosen primary that is a database secondary, and every chosen
+secondary that is one) gives one row [-1 at itself, nu at each species of
+its reaction], split into the columns of H2O + the chosen primaries and the
+columns of the chosen secondaries; with S^-1 the inverse of the square
+secondary block, the secondaries' stoichiometry in the chosen basis is
+-S^-1 P and their log K vectors (all database temperatures) -S^-1 logK.  A
+kinetic mineral's reaction then has each chosen secondary it names replaced
+by that secondary's reaction, log K += nu * log K (BasisSubSpeciesIn-
+MineralRxn, reaction_database_aux.F90:494-586; stoichiometries that cancel
+below 1e-10 are dropped).
 """
 from __future__ import annotations
 
@@ -251,40 +260,105 @@ class KineticMineralSpec:
     irreversible: int = 0
 
 
+def _species(db: Database, name: str) -> DbReaction:
+    for table in (db.primary, db.secondary):
+        if name in table:
+            return table[name]
+    raise KeyError("species %r not in the database" % name)
+
+
+def basis_reactions(db: Database, primaries: Sequence[str], secondaries: Sequence[str]):
+    """Reactions of the chosen secondaries in the chosen basis: a list, in
+    the order of `secondaries`, of (stoichiometry over H2O + primaries,
+    an array of len(primaries) + 1; log K over the database temperatures)."""
+    cols = [WATER] + list(primaries)
+    pcol = {n: i for i, n in enumerate(cols)}
+    scol = {n: i for i, n in enumerate(secondaries)}
+    for n in list(primaries) + list(secondaries):
+        _species(db, n)
+    rows = ([n for n in primaries if n in db.secondary] +
+            [n for n in secondaries if n in db.secondary])
+    ns = len(secondaries)
+    if len(rows) != ns:
+        raise ValueError("basis: %d species with database reactions for %d secondaries; "
+                         "the chosen primaries and secondaries are not a basis"
+                         % (len(rows), ns))
+    if ns == 0:
+        return []
+    P = np.zeros((ns, len(cols)))
+    S = np.zeros((ns, ns))
+    K = np.zeros((ns, db.temperatures.size))
+    for r, name in enumerate(rows):
+        rx = db.secondary[name]
+        K[r] = rx.logK
+        if name in pcol:
+            P[r, pcol[name]] -= 1.0
+        else:
+            S[r, scol[name]] -= 1.0
+        for sp, nu in zip(rx.species, rx.stoich):
+            if sp in pcol:
+                P[r, pcol[sp]] += nu
+            elif sp in scol:
+                S[r, scol[sp]] += nu
+            else:
+                raise ValueError("reaction %r names %r, which is neither a chosen primary "
+                                 "nor a chosen secondary species" % (name, sp))
+    Sinv = np.linalg.inv(S)
+    stoich = -Sinv @ P
+    logk = -Sinv @ K
+    return [(stoich[i], logk[i]) for i in range(ns)]
+
+
+def _in_basis(rx: DbReaction, cols, sec_rxn):
+    """a mineral reaction in the chosen basis: (stoichiometry over H2O +
+    primaries, log K over the database temperatures)"""
+    pcol = {n: i for i, n in enumerate(cols)}
+    st = np.zeros(len(cols))
+    logk = np.array(rx.logK, dtype=np.float64)
+    for sp, nu in zip(rx.species, rx.stoich):
+        if sp in pcol:
+            st[pcol[sp]] += nu
+        elif sp in sec_rxn:
+            s_st, s_lk = sec_rxn[sp]
+            st += nu * s_st
+            logk = logk + nu * s_lk
+        else:
+            raise ValueError("reaction %r names %r, which is neither a chosen primary nor a "
+                             "chosen secondary species" % (rx.name, sp))
+    st[np.abs(st) <= 1e-10] = 0.0
+    return st, logk
+
+
 def build_network(db: Database, primaries: Sequence[str], secondaries: Sequence[str] = (),
                   kinetic_minerals: Sequence[KineticMineralSpec] = (),
                   temperature: float = 25.0, isothermal: bool = False) -> ReactionNetwork:
-    """ReactionNetwork of the chosen species, 0-based ids in the order given"""
-    pid = {name: i for i, name in enumerate(primaries)}
+    """ReactionNetwork of the chosen species, 0-based ids in the order given
+    (any basis of the database reactions, see the module docstring)"""
+    cols = [WATER] + list(primaries)
     for name in primaries:
-        if name not in db.primary:
+        if name not in db.primary and name not in db.secondary:
             raise KeyError("primary species %r not in the database" % name)
+    sec = basis_reactions(db, primaries, secondaries)
+    sec_rxn = dict(zip(secondaries, sec))
 
-    def expand(rx: DbReaction):
-        sp, nu, h2o = [], [], 0.0
-        for s, v in zip(rx.species, rx.stoich):
-            if s == WATER and WATER not in pid:
-                h2o += v
-            elif s in pid:
-                sp.append(pid[s])
-                nu.append(v)
-            else:
-                raise ValueError("reaction %r names %r, which is not a chosen primary "
-                                 "species (basis swapping is not supported)" % (rx.name, s))
-        return sp, nu, h2o
+    def split(st):
+        """(species ids, stoichiometry, H2O coefficient) of a basis row"""
+        sp = [j - 1 for j in range(1, len(cols)) if abs(st[j]) > 1e-40]
+        return sp, [float(st[i + 1]) for i in sp], float(st[0])
 
     c_sp, c_nu, c_h2o, c_logK, c_coef, c_Z, c_a0 = [], [], [], [], [], [], []
-    for name in secondaries:
-        rx = db.secondary[name]
-        sp, nu, h2o = expand(rx)
-        lk, cf = logk_at(db.temperatures, rx.logK, temperature, isothermal)
+    for name, (st, lkv) in zip(secondaries, sec):
+        rx = _species(db, name)
+        sp, nu, h2o = split(st)
+        lk, cf = logk_at(db.temperatures, lkv, temperature, isothermal)
         c_sp.append(sp), c_nu.append(nu), c_h2o.append(h2o)
         c_logK.append(lk), c_coef.append(cf), c_Z.append(rx.Z), c_a0.append(rx.a0)
     m_sp, m_nu, m_h2o, m_logK, m_coef, m_vm = [], [], [], [], [], []
     for spec in kinetic_minerals:
         rx = db.minerals[spec.name]
-        sp, nu, h2o = expand(rx)
-        lk, cf = logk_at(db.temperatures, rx.logK, temperature, isothermal)
+        st, lkv = _in_basis(rx, cols, sec_rxn)
+        sp, nu, h2o = split(st)
+        lk, cf = logk_at(db.temperatures, lkv, temperature, isothermal)
         m_sp.append(sp), m_nu.append(nu), m_h2o.append(h2o)
         m_logK.append(lk), m_coef.append(cf), m_vm.append(rx.molar_volume * 1.0e-6)
     nm = len(kinetic_minerals)
@@ -297,8 +371,8 @@ def build_network(db: Database, primaries: Sequence[str], secondaries: Sequence[
 
     net = ReactionNetwork(
         primary_species_names=list(primaries),
-        primary_spec_Z=[db.primary[n].Z for n in primaries],
-        primary_spec_a0=[db.primary[n].a0 for n in primaries],
+        primary_spec_Z=[_species(db, n).Z for n in primaries],
+        primary_spec_a0=[_species(db, n).a0 for n in primaries],
         eqcplx_names=list(secondaries),
         eqcplx_species=c_sp, eqcplx_stoich=c_nu,
         eqcplxh2ostoich=np.array(c_h2o) if secondaries and any(c_h2o) else None,
