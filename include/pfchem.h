@@ -69,8 +69,15 @@ enum {
                                           (reaction.F90:4421-4431,4477-4486) */
   PFCHEM_CELL_SENS_UNSUPPORTED = 4,    /* pfchem_tangent/adjoint: the cell is
                                           dry (no sensitivity computed)      */
-  PFCHEM_CELL_TOO_MANY_SUBSTEPS = 5    /* pfchem_tangent/adjoint: more than 8
+  PFCHEM_CELL_TOO_MANY_SUBSTEPS = 5,   /* pfchem_tangent/adjoint: more than 8
                                           accepted sub-steps to record       */
+  PFCHEM_CELL_CONSTRAINT_SINGULAR = 6, /* pfchem_equilibrate: zero Jacobian
+                                          row (reaction.F90:1916-1926)       */
+  PFCHEM_CELL_CONSTRAINT_NONPOSITIVE = 7, /* pfchem_equilibrate: a free-ion
+                                          molality <= 0 (reaction.F90:1954-1975) */
+  PFCHEM_CELL_CONSTRAINT_MAX_ITERATIONS = 8 /* pfchem_equilibrate: no
+                                          convergence within max_iterations
+                                          (reaction.F90:2006-2016)           */
 };
 
 /* activity coefficient update frequency / algorithm
@@ -387,6 +394,74 @@ int pfchem_tangent(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
 int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double dt,
                    const double* w_vf, const double* w_sec, int jacobian, double* grad,
                    double* grad_cell, double* f_cell, int32_t* status, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* constraint speciation                                               */
+/* ------------------------------------------------------------------ */
+/* constraint types, same values as transport_constraint_rt.F90:22-34.
+   Supported here: NULL, FREE, TOTAL, LOG, PH, MINERAL, GAS, CHARGE_BAL.
+   PE/EH (redox), TOTAL_SORB / TOTAL_AQ_PLUS_SORB (sorption) and
+   SUPERCRIT_CO2 (CO2 equation of state) give PFCHEM_ERR_UNSUPPORTED. */
+enum {
+  PFCHEM_CONSTRAINT_NULL = 0,
+  PFCHEM_CONSTRAINT_FREE = 1,
+  PFCHEM_CONSTRAINT_TOTAL = 2,
+  PFCHEM_CONSTRAINT_LOG = 3,
+  PFCHEM_CONSTRAINT_PH = 4,
+  PFCHEM_CONSTRAINT_PE = 5,
+  PFCHEM_CONSTRAINT_EH = 6,
+  PFCHEM_CONSTRAINT_MINERAL = 7,
+  PFCHEM_CONSTRAINT_GAS = 8,
+  PFCHEM_CONSTRAINT_CHARGE_BAL = 9,
+  PFCHEM_CONSTRAINT_TOTAL_SORB = 10,
+  PFCHEM_CONSTRAINT_SUPERCRIT_CO2 = 11,
+  PFCHEM_CONSTRAINT_TOTAL_AQ_PLUS_SORB = 12
+};
+
+/* One aqueous constraint (tran_constraint_rt_type, aq_species_constraint
+   part) applied to many cells.  MINERAL and GAS constraints name a row of
+   the reaction table below: the mineral's (mineral_reaction%mnrl*) or
+   passive gas's (reaction%gas%paseq*) reaction written in the network's
+   basis, same compressed layout as the complexes. */
+typedef struct pfchem_constraint {
+  const int32_t* constraint_type; /* [naqcomp] PFCHEM_CONSTRAINT_*          */
+  const int32_t* constraint_rxn;  /* [naqcomp] row for MINERAL / GAS, else
+                                     ignored (constraint_spec_id)           */
+  int32_t nrxn;                   /* rows of the reaction table            */
+  int32_t rxn_maxsp;
+  const int32_t* rxnspecid;       /* [nrxn][rxn_maxsp+1]; [r][0] = count     */
+  const double* rxnstoich;        /* [nrxn][rxn_maxsp]                      */
+  const double* rxnh2ostoich;     /* [nrxn] or NULL                         */
+  const double* rxn_logK;         /* [nrxn] (used when isothermal)          */
+  const double* rxn_logKcoef;     /* [nrxn][5] or NULL (ReactionInterpolateLogK
+                                     coefficients, non-isothermal runs)     */
+  int32_t h_ion_id;               /* species_idx%h_ion_id: >= 0 H+ is that
+                                     primary; -(k+1): H+ is complex k; only
+                                     read when a PH constraint is present   */
+  int32_t initialize_with_molality; /* reaction%initialize_with_molality   */
+  int32_t use_prev_soln_as_guess; /* start from state->pri_molal           */
+  int32_t max_iterations;         /* the reference stops at 10000          */
+  double max_dlnC;                /* reaction%max_dlnC (default 5)          */
+  const double* free_ion_guess;   /* [naqcomp] or NULL: 1e-9 (host array)   */
+} pfchem_constraint;
+
+/* ReactionEquilibrateConstraint (reaction.F90:1328-2117) on ncell cells:
+   the free-ion molalities that satisfy the constraint, with
+   conc[i * state->ld + cell] the constraint value of component i in that
+   cell (device array; units as in the reference: mol/L for TOTAL unless
+   initialize_with_molality, pH, log10 activity, log10 or partial pressure
+   for GAS, a guess for MINERAL / CHARGE_BAL).  Writes pri_molal, total,
+   sec_molal, pri_act_coef, sec_act_coef, ln_act_h2o, num_iterations and
+   status (PFCHEM_CELL_OK or PFCHEM_CELL_CONSTRAINT_*); reads den_kg,
+   temperature (non-isothermal) and, with use_prev_soln_as_guess,
+   pri_molal.  The basis molarity of the reference is pri_molal *
+   den_kg / 1000.  Mineral volume fractions and areas of a constraint are
+   plain state values the caller writes.  No sorption, so the sorbed
+   totals the reference computes afterwards do not arise.  Blocking:
+   returns after the kernel finished on `stream`; PFCHEM_ERR_CELLS if a
+   cell failed. */
+int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                       const pfchem_constraint* con, const double* conc, void* stream);
 
 #ifdef __cplusplus
 }

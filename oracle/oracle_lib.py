@@ -304,3 +304,22 @@ def tangent_batch(net, opt, state, dt, wv, ws, dp, exact=True, nthreads=0):
                                   1 if exact else 0, nthreads, df.ctypes.data, f.ctypes.data,
                                   st.ctypes.data)
     return df, f, st
+
+
+def equilibrate_batch(net, opt, con, state, conc, nthreads=0):
+    """ReactionEquilibrateConstraint over all cells of a host CellState (in
+    place); conc: (naqcomp, ld) constraint values.  Returns #failed."""
+    L = lib()
+    if not getattr(L, "_con_set", False):
+        L.orc_equilibrate_batch.restype = C.c_int64
+        L.orc_equilibrate_batch.argtypes = [C.c_void_p] * 5 + [C.c_int64, C.c_int]
+        L._con_set = True
+    n = net.to_abi()
+    o = opt.to_abi()
+    c = con.to_abi(net)
+    s = state.to_abi()
+    cv = np.ascontiguousarray(conc, dtype=np.float64)
+    assert cv.shape[0] == net.naqcomp and cv.shape[1] >= state.ncell
+    assert cv.shape[1] == state.ld
+    return L.orc_equilibrate_batch(C.byref(n), C.byref(o), C.byref(c), C.byref(s),
+                                   cv.ctypes.data, state.ncell, nthreads)

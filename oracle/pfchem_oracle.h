@@ -139,6 +139,20 @@ double orc_record_rel_residual(const pfchem_network* net,
                                const orc_substep_record* rec);
 size_t orc_sizeof_record(void);
 
+/* ---- constraint speciation (pfchem_oracle_constraint.c) ---- */
+/* ReactionEquilibrateConstraint for one cell: conc[naqcomp] constraint
+   values.  Returns PFCHEM_CELL_OK or PFCHEM_CELL_CONSTRAINT_*, -1 for a
+   constraint type not restated. */
+int orc_equilibrate(const pfchem_network* net, const pfchem_options* opt,
+                    const pfchem_constraint* con, const double* conc, orc_cell* c,
+                    int* num_iterations);
+/* over cells [0, ncell); conc SoA with the state's ld; without
+   use_prev_soln_as_guess every cell starts with unit activity
+   coefficients.  Returns the number of failed cells. */
+int64_t orc_equilibrate_batch(const pfchem_network* net, const pfchem_options* opt,
+                              const pfchem_constraint* con, const pfchem_state* s,
+                              const double* conc, int64_t ncell, int nthreads);
+
 /* ---- sensitivities of one step (pfchem_oracle_adjoint.c) ----
    Objective of a cell: f = sum_m wv[m] VF_m(end) + sum_k ws[k] m_k(end)
    (end-of-step mineral volume fractions and secondary molalities).

@@ -45,6 +45,9 @@ struct pfchem_ctx {
   hipEvent_t sens_copied = nullptr;
   void* sens_work = nullptr;     // per-cell gradients + status when not given
   size_t sens_work_bytes = 0;
+  // constraint speciation: DevConstraint + reaction table + failure count
+  void* con_buf = nullptr;
+  size_t con_bytes = 0;
   std::string err;
 };
 
@@ -547,6 +550,7 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->sens_sum_h) (void)hipHostFree(ctx->sens_sum_h);
   if (ctx->sens_copied) (void)hipEventDestroy(ctx->sens_copied);
   if (ctx->sens_work) (void)hipFree(ctx->sens_work);
+  if (ctx->con_buf) (void)hipFree(ctx->con_buf);
   delete ctx;
   return PFCHEM_OK;
 }
@@ -947,6 +951,143 @@ int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
   if (e != hipSuccess) return hip_fail(ctx, e, "adjoint");
   if (grad)
     for (int q = 0; q < np; ++q) grad[q] = ctx->sens_sum_h[q];
+  return PFCHEM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// ReactionEquilibrateConstraint over ncell cells (reaction.F90:1328-2117)
+int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                       const pfchem_constraint* con, const double* conc, void* stream) {
+  if (!ctx) return PFCHEM_ERR_INVALID;
+  if (ncell < 0) return fail(ctx, PFCHEM_ERR_INVALID, "ncell < 0");
+  if (!con || !con->constraint_type || !con->constraint_rxn)
+    return fail(ctx, PFCHEM_ERR_INVALID, "null constraint");
+  if (ncell > 0 && !conc) return fail(ctx, PFCHEM_ERR_INVALID, "null conc");
+  const DevNet& n = ctx->dnet;
+  if (n.variable)
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "constraint speciation with per-cell active counts");
+  if (n.naq > kConMaxComp)
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "constraint speciation above 64 primaries");
+  if (con->max_iterations < 1 || !(con->max_dlnC > 0.0))
+    return fail(ctx, PFCHEM_ERR_INVALID, "max_iterations < 1 or max_dlnC <= 0");
+  const int nr = con->nrxn, ms = con->rxn_maxsp;
+  if (nr < 0 || (nr > 0 && (ms < 1 || !con->rxnspecid || !con->rxnstoich || !con->rxn_logK)))
+    return fail(ctx, PFCHEM_ERR_INVALID, "constraint reaction table");
+  for (int r = 0; r < nr; ++r) {
+    const int32_t* ids = con->rxnspecid + (size_t)r * (ms + 1);
+    if (ids[0] < 0 || ids[0] > ms)
+      return fail(ctx, PFCHEM_ERR_INVALID, "constraint reaction species count");
+    for (int j = 0; j < ids[0]; ++j)
+      if (ids[1 + j] < 0 || ids[1 + j] >= n.naq)
+        return fail(ctx, PFCHEM_ERR_INVALID, "constraint reaction species id");
+  }
+  for (int i = 0; i < n.naq; ++i) {
+    const int t = con->constraint_type[i];
+    switch (t) {
+      case PFCHEM_CONSTRAINT_NULL:
+      case PFCHEM_CONSTRAINT_FREE:
+      case PFCHEM_CONSTRAINT_TOTAL:
+      case PFCHEM_CONSTRAINT_LOG:
+      case PFCHEM_CONSTRAINT_CHARGE_BAL:
+        break;
+      case PFCHEM_CONSTRAINT_PH:
+        if (!(con->h_ion_id >= 0 ? con->h_ion_id < n.naq
+                                 : (-con->h_ion_id - 1) < n.nc && con->h_ion_id > -(1 << 30)))
+          return fail(ctx, PFCHEM_ERR_INVALID, "pH constraint, but H+ is not among the species");
+        break;
+      case PFCHEM_CONSTRAINT_MINERAL:
+      case PFCHEM_CONSTRAINT_GAS:
+        if (con->constraint_rxn[i] < 0 || con->constraint_rxn[i] >= nr)
+          return fail(ctx, PFCHEM_ERR_INVALID, "mineral / gas constraint without a reaction row");
+        break;
+      case PFCHEM_CONSTRAINT_PE:
+      case PFCHEM_CONSTRAINT_EH:
+      case PFCHEM_CONSTRAINT_TOTAL_SORB:
+      case PFCHEM_CONSTRAINT_SUPERCRIT_CO2:
+      case PFCHEM_CONSTRAINT_TOTAL_AQ_PLUS_SORB:
+        return fail(ctx, PFCHEM_ERR_UNSUPPORTED,
+                    "redox, sorption and supercritical-CO2 constraints are not supported");
+      default:
+        return fail(ctx, PFCHEM_ERR_INVALID, "unknown constraint type");
+    }
+  }
+  DevState d;
+  int rc = to_dev_state(ctx, state, ncell, &d);
+  if (rc) return rc;
+  if (ncell == 0) return PFCHEM_OK;
+  // one buffer: DevConstraint | ids | stoich | h2o | logK | coef | nfail
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const int rows = nr > 0 ? nr : 1, rms = nr > 0 ? ms : 1;
+  const size_t o_ids = up(sizeof(DevConstraint));
+  const size_t o_st = o_ids + up(sizeof(int) * rows * (rms + 1));
+  const size_t o_h2o = o_st + up(sizeof(double) * rows * rms);
+  const size_t o_lk = o_h2o + up(sizeof(double) * rows);
+  const size_t o_cf = o_lk + up(sizeof(double) * rows);
+  const size_t o_nf = o_cf + up(sizeof(double) * rows * 5);
+  const size_t bytes = o_nf + 256;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  hipStream_t s = (hipStream_t)stream;
+  if (bytes > ctx->con_bytes) {
+    if (ctx->con_buf) {
+      e = hipFree(ctx->con_buf);  // the previous call synchronized before returning
+      ctx->con_buf = nullptr;
+      ctx->con_bytes = 0;
+      if (e != hipSuccess) return hip_fail(ctx, e, "hipFree");
+    }
+    e = hipMalloc(&ctx->con_buf, bytes);
+    if (e != hipSuccess) {
+      ctx->con_buf = nullptr;
+      return fail(ctx, PFCHEM_ERR_NOMEM, "constraint buffer");
+    }
+    ctx->con_bytes = bytes;
+  }
+  std::vector<char> h(bytes, 0);
+  char* dbase = (char*)ctx->con_buf;
+  DevConstraint k{};
+  for (int i = 0; i < n.naq; ++i) {
+    k.type[i] = con->constraint_type[i];
+    k.rxn[i] = con->constraint_rxn[i];
+    k.guess[i] = con->free_ion_guess ? con->free_ion_guess[i] : 1.0e-9;
+  }
+  k.nrxn = nr;
+  k.ms = rms;
+  k.h_ion_id = con->h_ion_id;
+  k.with_molality = con->initialize_with_molality != 0;
+  k.use_prev = con->use_prev_soln_as_guess != 0;
+  k.max_it = con->max_iterations;
+  k.has_guess = con->free_ion_guess != nullptr;
+  k.has_h2o = nr > 0 && con->rxnh2ostoich != nullptr;
+  k.has_coef = nr > 0 && con->rxn_logKcoef != nullptr;
+  k.max_dlnC = con->max_dlnC;
+  k.ids = (const int*)(dbase + o_ids);
+  k.st = (const double*)(dbase + o_st);
+  k.h2o = (const double*)(dbase + o_h2o);
+  k.logK = (const double*)(dbase + o_lk);
+  k.coef = (const double*)(dbase + o_cf);
+  k.nfail = (int*)(dbase + o_nf);
+  std::memcpy(h.data(), &k, sizeof(k));
+  if (nr > 0) {
+    std::memcpy(h.data() + o_ids, con->rxnspecid, sizeof(int) * nr * (ms + 1));
+    std::memcpy(h.data() + o_st, con->rxnstoich, sizeof(double) * nr * ms);
+    if (k.has_h2o) std::memcpy(h.data() + o_h2o, con->rxnh2ostoich, sizeof(double) * nr);
+    std::memcpy(h.data() + o_lk, con->rxn_logK, sizeof(double) * nr);
+    if (k.has_coef) std::memcpy(h.data() + o_cf, con->rxn_logKcoef, sizeof(double) * nr * 5);
+  }
+  e = hipMemcpyAsync(ctx->con_buf, h.data(), bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "constraint copy");
+  e = launch_equilibrate(n, ctx->dnet_dev, (const DevConstraint*)ctx->con_buf, d, conc, ncell, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "equilibrate launch");
+  int nfail = 0;
+  e = hipMemcpyAsync(&nfail, dbase + o_nf, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "failure count copy");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "equilibrate");
+  if (nfail > 0)
+    return fail(ctx, PFCHEM_ERR_CELLS, std::to_string(nfail) + " cell(s) failed to equilibrate");
   return PFCHEM_OK;
 }
 

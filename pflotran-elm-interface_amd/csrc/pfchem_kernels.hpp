@@ -17,6 +17,29 @@ struct LaunchCfg {
 };
 
 int plain_max_ncomp();
+
+// One aqueous constraint as the constraint-speciation kernel reads it
+// (pfchem_constraint with 0-based ids); lives in one device buffer followed
+// by the reaction table its pointers address.
+constexpr int kConMaxComp = 64;
+struct DevConstraint {
+  int type[kConMaxComp];
+  int rxn[kConMaxComp];
+  double guess[kConMaxComp];
+  int nrxn, ms, h_ion_id, with_molality, use_prev, max_it, has_guess, has_h2o, has_coef;
+  double max_dlnC;
+  const int* ids;      // [nrxn][ms+1], [r][0] = count
+  const double* st;    // [nrxn][ms]
+  const double* h2o;   // [nrxn] (has_h2o)
+  const double* logK;  // [nrxn]
+  const double* coef;  // [nrxn][5] (has_coef)
+  int* nfail;          // cells that failed (zero on entry)
+};
+// ReactionEquilibrateConstraint, one cell per thread, FP64;
+// conc[i * s.ld + c] the constraint values of cell c
+hipError_t launch_equilibrate(const DevNet& n, const DevNet* dn, const DevConstraint* dk,
+                              const DevState& s, const double* conc, int64_t ncell,
+                              hipStream_t stream);
 hipError_t launch_plain_step(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                              double dt, hipStream_t stream);
 hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,

@@ -502,6 +502,224 @@ activity_kernel(const DevNet* np, DevState s, int64_t ncell) {
   if (s.ln_act_h2o && n.act_h2o) s.ln_act_h2o[c] = w.ln_act_h2o;
 }
 
+
+// ln Q/K of a constraint reaction row (mineral reaction.F90:1756-1771, gas
+// :1780-1793) at the cell's molalities and activity coefficients
+template <int MAXN>
+__device__ double con_lnqk(const DevNet& n, const DevConstraint& k, const PlainCell<MAXN>& w,
+                           int r) {
+  double logK = k.logK[r];
+  if (!n.isothermal && k.has_coef) logK = interp_logk(k.coef + 5 * r, w.temp);
+  double lnQK = -logK * kLogToLn;
+  if (k.has_h2o && k.h2o[r] != 0.0) lnQK += k.h2o[r] * w.ln_act_h2o;
+  const int* ids = k.ids + r * (k.ms + 1);
+  for (int j = 0; j < ids[0]; ++j) {
+    const int id = ids[1 + j];
+    lnQK += k.st[r * k.ms + j] * log(w.x[id] * w.gp[id]);
+  }
+  return lnQK;
+}
+
+// ReactionEquilibrateConstraint (reaction.F90:1328-2117) for one cell per
+// thread: constraint set-up, then the Newton loop with the two-stage
+// convergence test (activity coefficients switched on once the unit-
+// activity problem has converged).  J holds the naq x naq constraint
+// Jacobian (row stride naq).
+template <int MAXN>
+__global__ void __launch_bounds__(128)
+equilibrate_kernel(const DevNet* np, const DevConstraint* kp, DevState s, const double* conc,
+                   int64_t ncell) {
+  const DevNet& n = *np;
+  const DevConstraint& k = *kp;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int naq = n.naq;
+  PlainCell<MAXN> w;
+  w.naq = naq;
+  w.N = naq;
+  w.den = s.den_kg[c];
+  w.temp = s.temperature ? s.temperature[c] : 25.0;
+  double cv[MAXN], total_conc[MAXN], free_conc[MAXN], prev[MAXN];
+  for (int i = 0; i < naq; ++i) cv[i] = conc[i * s.ld + c];
+  // :1465-1472 (xmass = 1)
+  const double to_molar = k.with_molality ? w.den / 1000.0 : 1.0;
+  const double to_molal = k.with_molality ? 1.0 : 1000.0 / w.den;
+  int status = PFCHEM_CELL_OK, it = 0;
+  if (!n.full_geochem) {  // :1474-1478
+    for (int i = 0; i < naq; ++i) {
+      s.pri_molal[i * s.ld + c] = cv[i] * to_molal;
+      s.total[i * s.ld + c] = cv[i] * to_molar;
+    }
+    if (s.num_iterations) s.num_iterations[c] = 0;
+    if (s.status) s.status[c] = status;
+    return;
+  }
+  // a fresh auxvar has unit activity coefficients; with the previous
+  // solution as guess the state's coefficients are kept
+  if (k.use_prev) {
+    for (int i = 0; i < naq; ++i) w.gp[i] = s.pri_act_coef[i * s.ld + c];
+    w.ln_act_h2o = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
+  } else {
+    for (int i = 0; i < naq; ++i) w.gp[i] = 1.0;
+    for (int q = 0; q < n.nc; ++q) s.sec_act_coef[q * s.ld + c] = 1.0;
+    w.ln_act_h2o = 0.0;
+  }
+  // :1486-1599
+  for (int i = 0; i < naq; ++i) {
+    free_conc[i] = k.use_prev ? s.pri_molal[i * s.ld + c] : (k.has_guess ? k.guess[i] : 1.0e-9);
+    total_conc[i] = 0.0;
+    switch (k.type[i]) {
+      case PFCHEM_CONSTRAINT_NULL:
+      case PFCHEM_CONSTRAINT_TOTAL: total_conc[i] = cv[i] * to_molar; break;
+      case PFCHEM_CONSTRAINT_FREE: free_conc[i] = cv[i] * to_molal; break;
+      case PFCHEM_CONSTRAINT_LOG: free_conc[i] = pow(10.0, cv[i]) * to_molal; break;
+      case PFCHEM_CONSTRAINT_CHARGE_BAL:
+      case PFCHEM_CONSTRAINT_MINERAL:
+        if (!k.use_prev) free_conc[i] = cv[i] * to_molal;
+        break;
+      case PFCHEM_CONSTRAINT_PH: free_conc[i] = pow(10.0, -cv[i]); break;
+      case PFCHEM_CONSTRAINT_GAS:
+        if (cv[i] <= 0.0) cv[i] = pow(10.0, cv[i]);
+        break;
+      default: break;
+    }
+  }
+  for (int i = 0; i < naq; ++i) w.x[i] = free_conc[i];
+  int it_act_on = 0;
+  bool charge_warned = false, compute_act = k.use_prev != 0;
+  for (;;) {
+    for (int i = 0; i < naq; ++i)  // :1625-1631
+      if (k.type[i] == PFCHEM_CONSTRAINT_FREE || k.type[i] == PFCHEM_CONSTRAINT_LOG)
+        w.x[i] = free_conc[i];
+    if (n.act_freq != PFCHEM_ACT_COEF_FREQUENCY_OFF && compute_act)  // :1632-1638
+      plain_activity_lag(n, s, c, w);
+    plain_total(n, s, c, w, 1);  // :1639; dtotal (kg water / L) in J
+    // rows of J: TOTAL keeps its dtotal row; the others are rebuilt.  The
+    // charge-balance row needs every dtotal row, so it is built first.
+    double cb_row[MAXN];
+    bool has_cb = false;
+    for (int i = 0; i < naq; ++i) has_cb |= k.type[i] == PFCHEM_CONSTRAINT_CHARGE_BAL;
+    double cb_res = 0.0;
+    if (has_cb) {
+      for (int j = 0; j < naq; ++j) {
+        cb_res += n.Zp[j] * w.total[j];
+        double v = 0.0;
+        for (int q = 0; q < naq; ++q) v += n.Zp[q] * w.J[q * naq + j];
+        cb_row[j] = v;
+      }
+    }
+    for (int i = 0; i < naq; ++i) {
+      double* row = w.J + i * naq;
+      const int t = k.type[i];
+      if (t == PFCHEM_CONSTRAINT_NULL || t == PFCHEM_CONSTRAINT_TOTAL) {  // :1648-1655
+        w.res[i] = w.total[i] - total_conc[i];
+        continue;
+      }
+      for (int j = 0; j < naq; ++j) row[j] = 0.0;
+      w.res[i] = 0.0;
+      if (t == PFCHEM_CONSTRAINT_FREE || t == PFCHEM_CONSTRAINT_LOG) {  // :1680-1685
+        row[i] = 1.0;
+      } else if (t == PFCHEM_CONSTRAINT_CHARGE_BAL) {  // :1687-1714
+        w.res[i] = cb_res;
+        for (int j = 0; j < naq; ++j) row[j] = cb_row[j];
+        if (w.x[i] < 1.0e-20 && !charge_warned) {
+          const double z = n.Zp[i];
+          if ((cb_res > 0.0 && z > 0.0) || (cb_res < 0.0 && z < 0.0)) {
+            charge_warned = true;
+            w.x[i] = 1.0e-3;
+          }
+        }
+      } else if (t == PFCHEM_CONSTRAINT_PH) {  // :1716-1753
+        if (k.h_ion_id >= 0) {
+          w.x[i] = pow(10.0, -cv[i]) / w.gp[i];
+          row[i] = 1.0;
+        } else {
+          const int q = -k.h_ion_id - 1, ms = n.cplx_ms;
+          double logK = n.cplx_logK[q];
+          if (!n.isothermal && n.has_cplx_logKcoef)
+            logK = interp_logk(n.cplx_logKcoef + 5 * q, w.temp);
+          double lnQK = -logK * kLogToLn;
+          if (n.has_cplx_h2o && n.cplx_h2o[q] != 0.0) lnQK += n.cplx_h2o[q] * w.ln_act_h2o;
+          const int* ids = n.cplx_ids + q * (ms + 1);
+          for (int j = 0; j < ids[0]; ++j)
+            lnQK += n.cplx_st[q * ms + j] * log(w.x[ids[1 + j]] * w.gp[ids[1 + j]]);
+          lnQK += cv[i] * kLogToLn;
+          const double QK = exp(lnQK);
+          w.res[i] = 1.0 - QK;
+          for (int j = 0; j < ids[0]; ++j)
+            row[ids[1 + j]] = -QK / w.x[ids[1 + j]] * n.cplx_st[q * ms + j];
+        }
+      } else {  // MINERAL / GAS, :1754-1804
+        const int r = k.rxn[i];
+        const double lnQK = con_lnqk(n, k, w, r);
+        w.res[i] = t == PFCHEM_CONSTRAINT_MINERAL ? lnQK : lnQK - log(cv[i]);
+        const int* ids = k.ids + r * (k.ms + 1);
+        for (int j = 0; j < ids[0]; ++j) row[ids[1 + j]] = k.st[r * k.ms + j] / w.x[ids[1 + j]];
+      }
+    }
+    double max_res = 0.0;
+    for (int i = 0; i < naq; ++i) max_res = fmax(max_res, fabs(w.res[i]));
+    // :1870-1881
+    const bool use_log = n.log_form ? ((it > 3 && it < 9) ? (it % 2 == 0) : true) : false;
+    // RSolve (:5457-5516): row scaling, log-variable columns, LU
+    double upd[MAXN];
+    for (int i = 0; i < naq; ++i) {
+      double nrm = 0.0;
+      for (int j = 0; j < naq; ++j) nrm = fmax(nrm, fabs(w.J[i * naq + j]));
+      nrm = 1.0 / fmax(1.0, nrm);
+      upd[i] = w.res[i] * nrm;
+      for (int j = 0; j < naq; ++j) w.J[i * naq + j] *= nrm;
+    }
+    if (use_log)
+      for (int i = 0; i < naq; ++i)
+        for (int j = 0; j < naq; ++j) w.J[i * naq + j] *= w.x[j];
+    if (crout_lu_solve<MAXN>(w.J, naq, upd)) {
+      status = PFCHEM_CELL_CONSTRAINT_SINGULAR;
+      break;
+    }
+    for (int i = 0; i < naq; ++i) prev[i] = w.x[i];
+    if (use_log) {  // :1930-1932
+      for (int i = 0; i < naq; ++i) {
+        const double u = copysign(1.0, upd[i]) * fmin(fabs(upd[i]), k.max_dlnC);
+        w.x[i] *= exp(-u);
+      }
+    } else {  // :1933-1950
+      double min_ratio = 1.7976931348623157e308;
+      for (int i = 0; i < naq; ++i)
+        if (prev[i] <= upd[i]) min_ratio = fmin(min_ratio, fabs(prev[i] / upd[i]));
+      if (min_ratio <= 1.0)
+        for (int i = 0; i < naq; ++i) upd[i] *= min_ratio * 0.99;
+      for (int i = 0; i < naq; ++i) w.x[i] = prev[i] - upd[i];
+    }
+    double min_m = w.x[0], max_rel = 0.0;
+    for (int i = 1; i < naq; ++i) min_m = fmin(min_m, w.x[i]);
+    ++it;
+    if (min_m <= 0.0) {  // :1954-1975
+      status = PFCHEM_CELL_CONSTRAINT_NONPOSITIVE;
+      break;
+    }
+    for (int i = 0; i < naq; ++i) max_rel = fmax(max_rel, fabs((w.x[i] - prev[i]) / prev[i]));
+    if (it >= k.max_it) {  // :2002-2016
+      status = PFCHEM_CELL_CONSTRAINT_MAX_ITERATIONS;
+      break;
+    }
+    if (max_res < n.tol_abs && max_rel < n.tol_change) {  // :2017-2030
+      if (compute_act && it - it_act_on > 1) break;
+      if (!compute_act) it_act_on = it;
+      compute_act = true;
+    }
+  }
+  for (int i = 0; i < naq; ++i) {
+    s.pri_molal[i * s.ld + c] = w.x[i];
+    s.total[i * s.ld + c] = w.total[i];
+    s.pri_act_coef[i * s.ld + c] = w.gp[i];
+  }
+  if (s.ln_act_h2o) s.ln_act_h2o[c] = w.ln_act_h2o;
+  if (s.num_iterations) s.num_iterations[c] = it;
+  if (s.status) s.status[c] = status;
+  if (status != PFCHEM_CELL_OK) atomicAdd(k.nfail, 1);
+}
+
 }  // namespace
 
 int plain_max_ncomp() { return 64; }
@@ -534,6 +752,24 @@ hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState
     activity_kernel<16><<<grid, block, 0, stream>>>(dn, s, ncell);
   else
     activity_kernel<64><<<grid, block, 0, stream>>>(dn, s, ncell);
+  return hipGetLastError();
+}
+
+hipError_t launch_equilibrate(const DevNet& n, const DevNet* dn, const DevConstraint* dk,
+                              const DevState& s, const double* conc, int64_t ncell,
+                              hipStream_t stream) {
+  const int block = 128;
+  const dim3 grid((unsigned)((ncell + block - 1) / block));
+  if (n.naq <= 4)
+    equilibrate_kernel<4><<<grid, block, 0, stream>>>(dn, dk, s, conc, ncell);
+  else if (n.naq <= 8)
+    equilibrate_kernel<8><<<grid, block, 0, stream>>>(dn, dk, s, conc, ncell);
+  else if (n.naq <= 16)
+    equilibrate_kernel<16><<<grid, block, 0, stream>>>(dn, dk, s, conc, ncell);
+  else if (n.naq <= 32)
+    equilibrate_kernel<32><<<grid, block, 0, stream>>>(dn, dk, s, conc, ncell);
+  else
+    equilibrate_kernel<64><<<grid, block, 0, stream>>>(dn, dk, s, conc, ncell);
   return hipGetLastError();
 }
 

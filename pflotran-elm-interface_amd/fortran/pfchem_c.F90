@@ -22,6 +22,19 @@ module pfchem_c
   integer(c_int), parameter :: PFCHEM_CELL_MAX_CUTS_ITERATIONS = 1
   integer(c_int), parameter :: PFCHEM_CELL_MAX_CUTS_SINGULAR = 2
   integer(c_int), parameter :: PFCHEM_CELL_MAX_CUTS_ACTIVITY = 3
+  integer(c_int), parameter :: PFCHEM_CELL_CONSTRAINT_SINGULAR = 6
+  integer(c_int), parameter :: PFCHEM_CELL_CONSTRAINT_NONPOSITIVE = 7
+  integer(c_int), parameter :: PFCHEM_CELL_CONSTRAINT_MAX_ITERATIONS = 8
+
+  ! constraint types (transport_constraint_rt.F90:22-34)
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_NULL = 0
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_FREE = 1
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_TOTAL = 2
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_LOG = 3
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_PH = 4
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_MINERAL = 7
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_GAS = 8
+  integer(c_int), parameter :: PFCHEM_CONSTRAINT_CHARGE_BAL = 9
 
   ! activity coefficient options (the reaction_aux.F90:27-31 values)
   integer(c_int), parameter :: PFCHEM_ACT_COEF_FREQUENCY_OFF = 0
@@ -183,6 +196,25 @@ module pfchem_c
     integer(c_int32_t) :: rows_per_lane
   end type pfchem_launch
 
+  ! one aqueous constraint (pfchem_constraint); 0-based ids
+  type, bind(C) :: pfchem_constraint
+    type(c_ptr) :: constraint_type
+    type(c_ptr) :: constraint_rxn
+    integer(c_int32_t) :: nrxn
+    integer(c_int32_t) :: rxn_maxsp
+    type(c_ptr) :: rxnspecid
+    type(c_ptr) :: rxnstoich
+    type(c_ptr) :: rxnh2ostoich
+    type(c_ptr) :: rxn_logK
+    type(c_ptr) :: rxn_logKcoef
+    integer(c_int32_t) :: h_ion_id
+    integer(c_int32_t) :: initialize_with_molality
+    integer(c_int32_t) :: use_prev_soln_as_guess
+    integer(c_int32_t) :: max_iterations
+    real(c_double) :: max_dlnC
+    type(c_ptr) :: free_ion_guess
+  end type pfchem_constraint
+
   interface
     integer(c_int) function pfchem_create(net, opt, device, ctx) bind(C, name="pfchem_create")
       import :: c_int, c_ptr, pfchem_network, pfchem_options
@@ -242,6 +274,17 @@ module pfchem_c
       type(c_ptr), value :: jac
       type(c_ptr), value :: stream
     end function pfchem_residual_jacobian
+
+    integer(c_int) function pfchem_equilibrate(ctx, ncell, state, con, conc, stream) &
+        bind(C, name="pfchem_equilibrate")
+      import :: c_int, c_int64_t, c_ptr, pfchem_state, pfchem_constraint
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ncell
+      type(pfchem_state), intent(in) :: state
+      type(pfchem_constraint), intent(in) :: con
+      type(c_ptr), value :: conc
+      type(c_ptr), value :: stream
+    end function pfchem_equilibrate
 
     integer(c_int) function pfchem_set_launch(ctx, variant, cfg) bind(C, name="pfchem_set_launch")
       import :: c_int, c_ptr, pfchem_launch

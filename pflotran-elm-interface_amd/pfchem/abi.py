@@ -18,6 +18,26 @@ PFCHEM_CELL_OK = 0
 PFCHEM_CELL_MAX_CUTS_ITERATIONS = 1
 PFCHEM_CELL_MAX_CUTS_SINGULAR = 2
 PFCHEM_CELL_MAX_CUTS_ACTIVITY = 3
+PFCHEM_CELL_SENS_UNSUPPORTED = 4
+PFCHEM_CELL_TOO_MANY_SUBSTEPS = 5
+PFCHEM_CELL_CONSTRAINT_SINGULAR = 6
+PFCHEM_CELL_CONSTRAINT_NONPOSITIVE = 7
+PFCHEM_CELL_CONSTRAINT_MAX_ITERATIONS = 8
+
+# constraint types (transport_constraint_rt.F90:22-34)
+CONSTRAINT_NULL = 0
+CONSTRAINT_FREE = 1
+CONSTRAINT_TOTAL = 2
+CONSTRAINT_LOG = 3
+CONSTRAINT_PH = 4
+CONSTRAINT_PE = 5
+CONSTRAINT_EH = 6
+CONSTRAINT_MINERAL = 7
+CONSTRAINT_GAS = 8
+CONSTRAINT_CHARGE_BAL = 9
+CONSTRAINT_TOTAL_SORB = 10
+CONSTRAINT_SUPERCRIT_CO2 = 11
+CONSTRAINT_TOTAL_AQ_PLUS_SORB = 12
 
 # activity coefficients (reaction_aux.F90:27-31)
 ACT_COEF_FREQUENCY_OFF = 0
@@ -186,6 +206,26 @@ class State(C.Structure):
         ("num_sub_steps", C.c_void_p),
         ("status", C.c_void_p),
         ("ncomp_active", C.c_void_p),
+    ]
+
+
+class Constraint(C.Structure):
+    _fields_ = [
+        ("constraint_type", _ip),
+        ("constraint_rxn", _ip),
+        ("nrxn", C.c_int32),
+        ("rxn_maxsp", C.c_int32),
+        ("rxnspecid", _ip),
+        ("rxnstoich", _dp),
+        ("rxnh2ostoich", _dp),
+        ("rxn_logK", _dp),
+        ("rxn_logKcoef", _dp),
+        ("h_ion_id", C.c_int32),
+        ("initialize_with_molality", C.c_int32),
+        ("use_prev_soln_as_guess", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("max_dlnC", C.c_double),
+        ("free_ion_guess", _dp),
     ]
 
 

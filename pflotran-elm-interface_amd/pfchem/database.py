@@ -396,3 +396,28 @@ def build_network(db: Database, primaries: Sequence[str], secondaries: Sequence[
     )
     net.validate()
     return net
+
+
+def constraint_reactions(db: Database, primaries: Sequence[str], secondaries: Sequence[str],
+                         names: Sequence[str], temperature: float = 25.0,
+                         isothermal: bool = False):
+    """Mineral or passive-gas reactions of the database (its 'mineral' and
+    'gas' entries) written in the basis of `primaries`, for MINERAL and GAS
+    constraints (mnrlspecid / paseqspecid as BasisInit builds them,
+    reaction_database.F90:1800-1830, 1925-1960): one (species ids, stoichiometry, H2O
+    coefficient, log K at `temperature`, log K fit coefficients) per name."""
+    cols = [WATER] + list(primaries)
+    sec_rxn = dict(zip(secondaries, basis_reactions(db, primaries, secondaries)))
+    out = []
+    for name in names:
+        if name in db.minerals:
+            rx = db.minerals[name]
+        elif name in db.gases:
+            rx = db.gases[name]
+        else:
+            raise KeyError("%r is neither a mineral nor a gas of the database" % name)
+        st, lkv = _in_basis(rx, cols, sec_rxn)
+        sp = [j - 1 for j in range(1, len(cols)) if abs(st[j]) > 1e-40]
+        lk, cf = logk_at(db.temperatures, lkv, temperature, isothermal)
+        out.append((sp, [float(st[i + 1]) for i in sp], float(st[0]), lk, cf))
+    return out

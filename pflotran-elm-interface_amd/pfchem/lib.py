@@ -49,6 +49,7 @@ def load(path: str = LIB_PATH):
     L.pfchem_update_activity.argtypes = [vp, C.c_int64, vp, vp]
     L.pfchem_residual.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.pfchem_residual_jacobian.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, vp]
+    L.pfchem_equilibrate.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
     L.pfchem_set_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_get_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_tangent.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, vp, C.c_int, vp, vp, vp,
@@ -186,6 +187,26 @@ class Context:
                                                     ptr(fixed_immobile), dt, res.data_ptr(),
                                                     ptr(jac), sp), "pfchem_residual_jacobian")
         return res, jac
+
+    def equilibrate(self, st: DeviceState, con, conc, stream=None, check: bool = True):
+        """pfchem_equilibrate: constraint speciation of every cell of `st`.
+        con: constraint.Constraint; conc: device tensor (naqcomp, ld) of
+        constraint values.  Blocks until done.  With check, a cell that
+        failed raises PfchemError (its code is in st's status array);
+        otherwise the return code is returned."""
+        import torch
+        if conc.dtype != torch.float64 or not conc.is_cuda or not conc.is_contiguous():
+            raise ValueError("conc must be a contiguous float64 device tensor")
+        if tuple(conc.shape) != (self.net.naqcomp, st.ld):
+            raise ValueError("conc must have shape (naqcomp, ld)")
+        s = st.to_abi()
+        k = con.to_abi(self.net)
+        sp = (stream or torch.cuda.current_stream()).cuda_stream
+        rc = self.L.pfchem_equilibrate(self.h, st.ncell, C.byref(s), C.byref(k),
+                                       conc.data_ptr(), sp)
+        if check:
+            self._check(rc, "pfchem_equilibrate")
+        return rc
 
     def set_launch(self, variant, lanes_per_cell, block_size=256, blocks_per_cu=4,
                    rows_per_lane=1):

@@ -22,6 +22,13 @@ CASES = {
                                 "regression_tests/ascem/batch/carbonate.dat"),
     "calcite-kinetics": ("regression_tests/ascem/batch/calcite-kinetics.regression.gold",
                          "regression_tests/ascem/batch/calcite.dat"),
+    # initial state of a three-component deck: the constraint speciation's output
+    "ca-carbonate-debye-huckel-activity": ("regression_tests/ascem/batch/"
+                                           "ca-carbonate-debye-huckel-activity.regression.gold",
+                                           "regression_tests/ascem/batch/ca-carbonate.dat"),
+    "ca-carbonate-unit-activity": ("regression_tests/ascem/batch/"
+                                   "ca-carbonate-unit-activity.regression.gold",
+                                   "regression_tests/ascem/batch/ca-carbonate.dat"),
 }
 
 

@@ -55,10 +55,14 @@ def test_error_codes_without_a_device(lib):
     assert lib.pfchem_set_launch(None, abi.KERNEL_BATCHED_MIXED, C.byref(cfg)) == \
         abi.PFCHEM_ERR_INVALID
     assert lib.pfchem_create(None, None, 0, C.byref(C.c_void_p())) == abi.PFCHEM_ERR_INVALID
+    k = abi.Constraint()
+    assert lib.pfchem_equilibrate(None, 1, C.byref(st), C.byref(k), None, None) == \
+        abi.PFCHEM_ERR_INVALID
 
 
 STRUCTS = {"pfchem_network": abi.Network, "pfchem_options": abi.Options,
-           "pfchem_state": abi.State, "pfchem_launch": abi.Launch, "pfchem_somdec": abi.Somdec}
+           "pfchem_state": abi.State, "pfchem_launch": abi.Launch, "pfchem_somdec": abi.Somdec,
+           "pfchem_constraint": abi.Constraint}
 
 
 def test_ctypes_layout_matches_the_header():
