@@ -150,6 +150,37 @@ def main():
     log("[bench] girt residual+jacobian %.2f ms  %.3e cells/s" % (g_ms, n / (g_ms * 1e-3)))
     del gst_, fixed
 
+    # --- constraint speciation of every cell from its own totals
+    # (pfchem_equilibrate, TOTAL constraints, fresh guess each call), with
+    # the reference's default constraint tolerances (1e-12 absolute
+    # residual, 1e-6 relative change) and at most 500 iterations ---
+    import dataclasses
+    from pfchem import constraint as pcon
+    cctx = Context(net, dataclasses.replace(opt, max_residual_tolerance=1e-12,
+                                            max_relative_change_tolerance=1e-6))
+    cst = copies(1)[0]
+    con = pcon.Constraint(types=[abi.CONSTRAINT_TOTAL] * naq, rxn=[-1] * naq,
+                          max_iterations=500)
+    conc = cst.t["total"][:naq].clone().contiguous()
+    cctx.equilibrate(cst, con, conc, check=False)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        cctx.equilibrate(cst, con, conc, check=False)
+    e1.record()
+    torch.cuda.synchronize()
+    c_ms = e0.elapsed_time(e1) / reps
+    cgot = cst.to_host()
+    equil = {"kernel": "equilibrate_kernel (pfchem_equilibrate)", "ms": c_ms,
+             "cells_per_s": n / (c_ms * 1e-3),
+             "mean_iterations": float(cgot["num_iterations"][0, :n].mean()),
+             "failed_cells": int((cgot["status"][0, :n] != 0).sum())}
+    log("[bench] constraint speciation %.2f ms  %.3e cells/s  %.1f iterations  %d failed" %
+        (c_ms, n / (c_ms * 1e-3), equil["mean_iterations"], equil["failed_cells"]))
+    del cst, conc, cgot, cctx
+
     got = mix_state.to_host()
     status = got["status"][0, :n]
     iters_gpu = got["num_iterations"][0, :n]
@@ -263,6 +294,7 @@ def main():
         "batched_fp64_cells_per_s": b64_cps,
         "speedup_vs_plain_fp64": value / plain_cps,
         "girt_residual_jacobian": girt,
+        "constraint_speciation": equil,
         "failed_cells": int((status != 0).sum()),
         "newton_iterations_mean": float(iters_gpu.mean()),
     }
