@@ -181,6 +181,7 @@ contains
     type(pfchem_state), target :: st
     type(pfchem_somdec), target :: sd
     type(pfchem_launch), target :: la
+    type(pfchem_constraint), target :: con
     integer(c_intptr_t) :: b
     b = transfer(c_loc(net), b)
     write(*, '(*(g0,:," "))') 'network', c_sizeof(net), &
@@ -198,6 +199,10 @@ contains
     write(*, '(*(g0,:," "))') 'somdec', c_sizeof(sd), transfer(c_loc(sd%x0eps), b) - b, &
       transfer(c_loc(sd%rate_constant), b) - b, transfer(c_loc(sd%inhibition_constant2), b) - b
     write(*, '(*(g0,:," "))') 'launch', c_sizeof(la)
+    b = transfer(c_loc(con), b)
+    write(*, '(*(g0,:," "))') 'constraint', c_sizeof(con), transfer(c_loc(con%nrxn), b) - b, &
+      transfer(c_loc(con%h_ion_id), b) - b, transfer(c_loc(con%max_dlnC), b) - b, &
+      transfer(c_loc(con%free_ion_guess), b) - b
   end subroutine print_layout
 
 end program pfchem_rstep_driver

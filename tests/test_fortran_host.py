@@ -40,6 +40,9 @@ int main(void) {
   printf("somdec %zu %zu %zu %zu\n", sizeof(pfchem_somdec), offsetof(pfchem_somdec, x0eps),
          offsetof(pfchem_somdec, rate_constant), offsetof(pfchem_somdec, inhibition_constant2));
   printf("launch %zu\n", sizeof(pfchem_launch));
+  printf("constraint %zu %zu %zu %zu %zu\n", sizeof(pfchem_constraint),
+         offsetof(pfchem_constraint, nrxn), offsetof(pfchem_constraint, h_ion_id),
+         offsetof(pfchem_constraint, max_dlnC), offsetof(pfchem_constraint, free_ion_guess));
   return 0;
 }
 """
