@@ -349,19 +349,37 @@ int pfchem_residual_jacobian(pfchem_ctx* ctx, int64_t ncell, const pfchem_state*
                              const double* fixed_total, const double* fixed_immobile,
                              double dt, double* res, double* jac, void* stream);
 
-/* Launch configuration of the batched kernels, set by the autotuner: a
-   team of lanes_per_cell lanes owns one cell (64 / lanes_per_cell cells per
+/* Launch configuration of the batched kernels.  lanes_per_cell = 1: the
+   network-specialized kernel, one cell per lane (64 cells per wavefront),
+   generated from the network and options at pfchem_create and compiled for
+   the device at the first batched step; it takes aqueous networks of up to
+   16 primaries (plus immobile species with first-order decay) without the
+   SomDec sandbox or per-cell active counts, and is then the default.
+   block_size must be 256 and blocks_per_cu caps the workgroups per CU (0:
+   as many as its register use allows).  lanes_per_cell > 1: a team of
+   lanes_per_cell lanes owns one cell (64 / lanes_per_cell cells per
    wavefront) and each lane owns rows_per_lane rows of the cell's system
-   (lanes_per_cell * rows_per_lane >= ncomp).  Shapes compiled in:
+   (lanes_per_cell * rows_per_lane >= ncomp).  Team shapes compiled in:
    4x1 2x4 2x8 4x4 8x2 16x1 8x4 32x1 64x1. */
 typedef struct pfchem_launch {
-  int32_t lanes_per_cell;   /* 2, 4, 8, 16, 32 or 64                      */
+  int32_t lanes_per_cell;   /* 1 (one-lane kernel), 2, 4, 8, 16, 32, 64  */
   int32_t block_size;       /* threads per workgroup (64, 128 or 256)     */
-  int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CUs      */
+  int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * CUs (0:
+                               the kernel's default)                     */
   int32_t rows_per_lane;    /* 1, 2, 4 or 8                               */
 } pfchem_launch;
 int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg);
 int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg);
+
+/* Generate and compile the network-specialized one-lane kernels of
+   (net, opt) for gfx950 without a device, e.g. to check a network at set-up
+   time (pfchem_step compiles them on the device at its first batched call).
+   PFCHEM_ERR_UNSUPPORTED: the one-lane kernel does not take this network
+   (the team kernels run it).  source_path (optional) receives the generated
+   device source.  No reference counterpart: the reference interprets the
+   network at run time (reaction_aux.F90:123-311). */
+int pfchem_prepare_kernels(const pfchem_network* net, const pfchem_options* opt,
+                           const char* source_path);
 
 /* ------------------------------------------------------------------ */
 /* sensitivities of one step                                           */

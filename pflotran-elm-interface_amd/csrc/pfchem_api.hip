@@ -10,6 +10,7 @@
 #include "pfchem.h"
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
+#include "pfchem_lane.h"
 
 using namespace pfchem;
 
@@ -24,6 +25,8 @@ struct pfchem_ctx {
   DevNet* dnet_dev = nullptr;             // device copy of dnet
   DevState* dstate_ring = nullptr;        // device slots for per-launch state descriptors
   DevState* hstate_ring = nullptr;        // pinned host mirror the slots are copied from
+  hipEvent_t ring_done[256] = {};         // per slot: recorded after the launch reading it
+  bool ring_used[256] = {};               // the slot's event has been recorded
   int ring_next = 0;
   LaunchCfg launch[3] = {{1, 1, 128, 0}, {16, 1, 256, 4}, {16, 1, 256, 4}};
   // per-cell active counts: team shape per size bucket, per batched variant
@@ -48,6 +51,15 @@ struct pfchem_ctx {
   // constraint speciation: DevConstraint + reaction table + failure count
   void* con_buf = nullptr;
   size_t con_bytes = 0;
+  // network-specialized lane kernel (pfchem_lane.hpp): generated source,
+  // whether the network qualifies, whether it is compiled on the device
+  std::string lane_src;
+  bool lane_ok = false;
+  bool lane_ready = false;
+  std::string lane_why;
+  LaunchCfg team_default{16, 1, 256, 4};  // team shape of the FP64 rescue pass
+  int32_t* status_scratch = nullptr;      // per-cell status when the caller gives none
+  int64_t status_cap = 0;
   std::string err;
 };
 
@@ -74,9 +86,34 @@ struct Packer {
 };
 
 const size_t kNone = ~size_t(0);
-// launches take their state descriptor from a ring of device slots; a slot
-// is rewritten (stream-ordered) only after kStateRing further launches
+// Launches take their state descriptor from a ring of device slots, filled
+// from a pinned host mirror by a copy on the launch's stream.  An event is
+// recorded after every launch that reads a slot; before the slot (and its
+// host mirror) is rewritten, that event must have completed, whatever
+// stream the earlier launch used.
 const int kStateRing = 256;
+
+DevState* acquire_slot(pfchem_ctx* ctx, const DevState& d, hipStream_t s, int* idx,
+                       hipError_t* e) {
+  const int k = ctx->ring_next;
+  ctx->ring_next = (k + 1) % kStateRing;
+  *idx = k;
+  if (ctx->ring_used[k]) {
+    *e = hipEventSynchronize(ctx->ring_done[k]);
+    if (*e != hipSuccess) return nullptr;
+    ctx->ring_used[k] = false;
+  }
+  ctx->hstate_ring[k] = d;
+  *e = hipMemcpyAsync(ctx->dstate_ring + k, ctx->hstate_ring + k, sizeof(DevState),
+                      hipMemcpyHostToDevice, s);
+  return ctx->dstate_ring + k;
+}
+
+hipError_t release_slot(pfchem_ctx* ctx, int k, hipStream_t s) {
+  const hipError_t e = hipEventRecord(ctx->ring_done[k], s);
+  ctx->ring_used[k] = e == hipSuccess;
+  return e;
+}
 
 // SomDec descriptor: ids in range; the kernels take totals as free molality
 // times density (no complexes) and do not carry the Monod oxygen response
@@ -523,6 +560,21 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
       ctx->vlaunch[v - 1][b] = cb;
     }
   }
+  for (int k = 0; k < kStateRing; ++k)
+    if (hipEventCreateWithFlags(&ctx->ring_done[k], hipEventDisableTiming) != hipSuccess) {
+      pfchem_destroy(ctx);
+      return PFCHEM_ERR_HIP;
+    }
+  // the FP64 team shape also takes the cells the lane kernel hands over
+  ctx->team_default = ctx->launch[PFCHEM_KERNEL_BATCHED_FP64];
+  // network-specialized one-cell-per-lane kernel: the default batched
+  // kernel of every network it takes (compiled at the first step)
+  ctx->lane_ok = lane_supported(net, opt, ctx->lane_why) != 0;
+  if (ctx->lane_ok) {
+    ctx->lane_src = lane_source(net, opt);
+    for (int v = PFCHEM_KERNEL_BATCHED_FP64; v <= PFCHEM_KERNEL_BATCHED_MIXED; ++v)
+      ctx->launch[v] = LaunchCfg{1, 1, 256, 0, ctx->max_lds};
+  }
   if (hipMalloc(&ctx->dnet_dev, sizeof(DevNet)) != hipSuccess ||
       hipMalloc(&ctx->dstate_ring, sizeof(DevState) * kStateRing) != hipSuccess ||
       hipHostMalloc(&ctx->hstate_ring, sizeof(DevState) * kStateRing) != hipSuccess ||
@@ -551,6 +603,9 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->sens_copied) (void)hipEventDestroy(ctx->sens_copied);
   if (ctx->sens_work) (void)hipFree(ctx->sens_work);
   if (ctx->con_buf) (void)hipFree(ctx->con_buf);
+  if (ctx->status_scratch) (void)hipFree(ctx->status_scratch);
+  for (int k = 0; k < kStateRing; ++k)
+    if (ctx->ring_done[k]) (void)hipEventDestroy(ctx->ring_done[k]);
   delete ctx;
   return PFCHEM_OK;
 }
@@ -593,6 +648,81 @@ static int to_dev_state(pfchem_ctx* ctx, const pfchem_state* st, int64_t ncell, 
   d->num_iterations = st->num_iterations;
   d->num_sub_steps = st->num_sub_steps;
   d->status = st->status;
+  return PFCHEM_OK;
+}
+
+// Batched step on the network-specialized lane kernel: FP32 (mixed) or FP64
+// pass over every cell; for mixed an FP64 lane pass over the cells the FP32
+// pass handed over; then the FP64 team kernel (pivoting fallback) over the
+// cells still handed over.
+static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double dt, bool mixed,
+                     hipStream_t s) {
+  if (!ctx->lane_ok)
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "one-lane kernel: " + ctx->lane_why);
+  if (!ctx->lane_ready) {
+    std::string msg;
+    const int rc = lane_compile(ctx->lane_src, ctx->device, msg);
+    if (rc) return fail(ctx, rc, msg);
+    ctx->lane_ready = true;
+  }
+  int32_t* st = d.status;
+  if (!st) {
+    if (ncell > ctx->status_cap) {
+      if (ctx->status_scratch) (void)hipFree(ctx->status_scratch);
+      ctx->status_scratch = nullptr;
+      ctx->status_cap = 0;
+      if (hipMalloc(&ctx->status_scratch, sizeof(int32_t) * (size_t)ncell) != hipSuccess)
+        return fail(ctx, PFCHEM_ERR_NOMEM, "status scratch");
+      ctx->status_cap = ncell;
+    }
+    st = ctx->status_scratch;
+  }
+  LaneArgs a{};
+  a.ld = d.ld;
+  a.ncell = ncell;
+  a.pri_molal = d.pri_molal;
+  a.total = d.total;
+  a.immobile = d.immobile;
+  a.sec_molal = d.sec_molal;
+  a.pri_act_coef = d.pri_act_coef;
+  a.sec_act_coef = d.sec_act_coef;
+  a.ln_act_h2o = d.ln_act_h2o;
+  a.mnrl_volfrac = d.mnrl_volfrac;
+  a.mnrl_area = d.mnrl_area;
+  a.mnrl_rate = d.mnrl_rate;
+  a.porosity = d.porosity;
+  a.saturation = d.saturation;
+  a.den_kg = d.den_kg;
+  a.volume = d.volume;
+  a.temperature = d.temperature;
+  a.num_iterations = d.num_iterations;
+  a.num_sub_steps = d.num_sub_steps;
+  a.status = st;
+  a.dt = dt;
+  a.nxcd = ctx->nxcd;
+  a.escalate = 1;
+  const int bpc = ctx->launch[mixed ? PFCHEM_KERNEL_BATCHED_MIXED : PFCHEM_KERNEL_BATCHED_FP64]
+                      .blocks_per_cu;
+  hipError_t e;
+  if (mixed) {
+    a.rescue = 0;
+    e = lane_launch(ctx->lane_src, ctx->device, 1, a, ctx->num_cus, bpc, ctx->counter, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "lane kernel launch (FP32)");
+    a.rescue = 1;
+  } else {
+    a.rescue = 0;
+  }
+  e = lane_launch(ctx->lane_src, ctx->device, 0, a, ctx->num_cus, bpc, ctx->counter, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "lane kernel launch (FP64)");
+  DevState dr = d;
+  dr.status = st;
+  int k = 0;
+  DevState* slot = acquire_slot(ctx, dr, s, &k, &e);
+  if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+  e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, kTeamRescue64,
+                       ctx->team_default, ctx->counter, ctx->num_cus, ctx->nxcd, s, true);
+  if (e == hipSuccess) e = release_slot(ctx, k, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "team rescue launch");
   return PFCHEM_OK;
 }
 
@@ -643,29 +773,27 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
         DevState db = d;
         db.cell_list = ctx->buckets + (size_t)b * ncell;
         db.list_count = counts + b;
-        DevState* slot = ctx->dstate_ring + ctx->ring_next;
-        DevState* hslot = ctx->hstate_ring + ctx->ring_next;
-        ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
-        *hslot = db;
-        e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+        int k = 0;
+        DevState* slot = acquire_slot(ctx, db, s, &k, &e);
         if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
         e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cb, ctx->counter,
                              ctx->num_cus, ctx->nxcd, s, d.status != nullptr);
         if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+        e = release_slot(ctx, k, s);
+        if (e != hipSuccess) return hip_fail(ctx, e, "descriptor event");
       }
       return PFCHEM_OK;
     }
     const LaunchCfg& cfg = ctx->launch[variant];
+    if (cfg.lanes_per_cell == 1) return lane_step(ctx, ncell, d, dt, mixed, s);
     if (!team_supports(ctx->dnet, cfg, mixed))
       return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
-    DevState* slot = ctx->dstate_ring + ctx->ring_next;
-    DevState* hslot = ctx->hstate_ring + ctx->ring_next;
-    ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
-    *hslot = d;
-    e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+    int k = 0;
+    DevState* slot = acquire_slot(ctx, d, s, &k, &e);
     if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
     e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cfg, ctx->counter,
                          ctx->num_cus, ctx->nxcd, s, d.status != nullptr);
+    if (e == hipSuccess) e = release_slot(ctx, k, s);
   }
   if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
   return PFCHEM_OK;
@@ -701,15 +829,14 @@ int pfchem_residual_jacobian(pfchem_ctx* ctx, int64_t ncell, const pfchem_state*
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
   hipStream_t s = (hipStream_t)stream;
-  DevState* slot = ctx->dstate_ring + ctx->ring_next;
-  DevState* hslot = ctx->hstate_ring + ctx->ring_next;
-  ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
-  *hslot = d;
-  e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
+  int k = 0;
+  DevState* slot = acquire_slot(ctx, d, s, &k, &e);
   if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
   e = launch_girt_team(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, fixed_total, fixed_immobile, res,
                        jac, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "residual launch");
+  e = release_slot(ctx, k, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "descriptor event");
   return PFCHEM_OK;
 }
 
@@ -725,10 +852,24 @@ int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg) {
   if (variant < PFCHEM_KERNEL_PLAIN_FP64 || variant > PFCHEM_KERNEL_BATCHED_MIXED)
     return fail(ctx, PFCHEM_ERR_INVALID, "unknown kernel variant");
   if (cfg->block_size <= 0 || cfg->block_size % 64 || cfg->block_size > 1024 ||
-      cfg->blocks_per_cu < 1 || cfg->blocks_per_cu > 64)
+      cfg->blocks_per_cu < 0 || cfg->blocks_per_cu > 64)
     return fail(ctx, PFCHEM_ERR_INVALID, "bad launch shape");
+  // blocks_per_cu 0: the default (team kernels 4 per CU; the one-lane kernel
+  // as many as its occupancy allows)
   LaunchCfg c{cfg->lanes_per_cell, cfg->rows_per_lane > 0 ? cfg->rows_per_lane : 1,
-              cfg->block_size, cfg->blocks_per_cu, ctx->max_lds};
+              cfg->block_size, cfg->blocks_per_cu ? cfg->blocks_per_cu
+                                                  : (cfg->lanes_per_cell == 1 ? 0 : 4),
+              ctx->max_lds};
+  if (variant != PFCHEM_KERNEL_PLAIN_FP64 && c.lanes_per_cell == 1) {
+    // the network-specialized lane kernel: 256-lane workgroups, at most as
+    // many per CU as its occupancy allows
+    if (!ctx->lane_ok)
+      return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "one-lane kernel: " + ctx->lane_why);
+    if (c.rows_per_lane != 1 || c.block_size != 256)
+      return fail(ctx, PFCHEM_ERR_INVALID, "the one-lane kernel runs 256-lane workgroups, 1 row");
+    ctx->launch[variant] = c;
+    return PFCHEM_OK;
+  }
   if (variant != PFCHEM_KERNEL_PLAIN_FP64 &&
       !team_supports(ctx->dnet, c, variant == PFCHEM_KERNEL_BATCHED_MIXED))
     return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "launch configuration does not fit this network");
@@ -737,6 +878,31 @@ int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg) {
     for (int b = 0; b < kBuckets; ++b)
       if (kBucketBound[b] == c.lanes_per_cell) ctx->vlaunch[variant - 1][b] = c;
   return PFCHEM_OK;
+}
+
+int pfchem_prepare_kernels(const pfchem_network* net, const pfchem_options* opt,
+                           const char* source_path) {
+  if (!net || !opt) return PFCHEM_ERR_INVALID;
+  if (net->naqcomp < 1 || net->nimmobile < 0 ||
+      (net->neqcplx > 0 && (!net->eqcplxspecid || !net->eqcplxstoich || !net->eqcplx_logK)) ||
+      (net->nkinmnrl > 0 && (!net->kinmnrlspecid || !net->kinmnrlstoich ||
+                             !net->kinmnrl_logK || !net->kinmnrl_rate_constant ||
+                             !net->kinmnrl_molar_vol)) ||
+      !net->primary_spec_Z || !net->primary_spec_a0)
+    return PFCHEM_ERR_INVALID;
+  std::string why;
+  if (!lane_supported(net, opt, why)) return PFCHEM_ERR_UNSUPPORTED;
+  const std::string src = lane_source(net, opt);
+  if (source_path) {
+    FILE* f = std::fopen(source_path, "w");
+    if (!f) return PFCHEM_ERR_INVALID;
+    std::fwrite(src.data(), 1, src.size(), f);
+    std::fclose(f);
+  }
+  std::string err;
+  const int rc = lane_compile_only(src, nullptr, err);
+  if (rc) std::fprintf(stderr, "pfchem_prepare_kernels: %s\n", err.c_str());
+  return rc;
 }
 
 int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg) {
@@ -870,14 +1036,6 @@ int sens_check(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, double
   return PFCHEM_OK;
 }
 
-DevState* sens_state_slot(pfchem_ctx* ctx, const DevState& d, hipStream_t s, hipError_t* e) {
-  DevState* slot = ctx->dstate_ring + ctx->ring_next;
-  DevState* hslot = ctx->hstate_ring + ctx->ring_next;
-  ctx->ring_next = (ctx->ring_next + 1) % kStateRing;
-  *hslot = d;
-  *e = hipMemcpyAsync(slot, hslot, sizeof(DevState), hipMemcpyHostToDevice, s);
-  return slot;
-}
 
 }  // namespace
 
@@ -894,12 +1052,14 @@ int pfchem_tangent(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
   rc = sens_stage(ctx, w_vf, w_sec, dp, s);
   if (rc) return rc;
   hipError_t e;
-  DevState* slot = sens_state_slot(ctx, d, s, &e);
+  int k = 0;
+  DevState* slot = acquire_slot(ctx, d, s, &k, &e);
   if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
   const int np = ctx->dnet.nm + ctx->dnet.nc;
   e = launch_sens(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, ctx->sens_in,
                   ctx->sens_in + ctx->dnet.nm, ctx->sens_in + np, 0, jacobian ? 1 : 0, f_cell,
                   df_cell, ncell, status, nullptr, s);
+  if (e == hipSuccess) e = release_slot(ctx, k, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "tangent launch");
   return PFCHEM_OK;
 }
@@ -939,11 +1099,13 @@ int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
   }
   int32_t* st = status ? status : (int32_t*)wp;
   hipError_t e;
-  DevState* slot = sens_state_slot(ctx, d, s, &e);
+  int k = 0;
+  DevState* slot = acquire_slot(ctx, d, s, &k, &e);
   if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
   e = launch_sens(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, ctx->sens_in,
                   ctx->sens_in + ctx->dnet.nm, nullptr, 1, jacobian ? 1 : 0, f_cell, gc, ncell,
                   st, ctx->sens_sum, s);
+  if (e == hipSuccess) e = release_slot(ctx, k, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(ctx->sens_sum_h, ctx->sens_sum, np * sizeof(double),
                        hipMemcpyDeviceToHost, s);

@@ -66,11 +66,14 @@ constexpr bool team_has_variable(int L, int R) {
   return R == 1 && (L == 16 || L == 32 || L == 64);
 }
 int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed);
-// dn / ds: device copies of the network / state descriptors.  The mixed
-// variant runs the FP32 pass and then an FP64 pass over the cells the FP32
-// pass could not converge (needs the per-cell status array).
+// dn / ds: device copies of the network / state descriptors.  mixed = 0:
+// FP64; 1: the FP32 pass and then an FP64 pass over the cells the FP32
+// pass could not converge (needs the per-cell status array); kTeamRescue64:
+// only the FP64 pass over cells whose status is kTeamEscalate.
+constexpr int kTeamRescue64 = 2;
+constexpr int kTeamEscalate = 16;
 hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,
-                            int64_t ncell, double dt, bool mixed, const LaunchCfg& cfg,
+                            int64_t ncell, double dt, int mixed, const LaunchCfg& cfg,
                             unsigned long long* cursors, int num_cus, int nxcd,
                             hipStream_t stream, bool has_status);
 

@@ -5,39 +5,39 @@
 namespace pfchem {
 
 hipError_t launch_team_4_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_2_4(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_2_8(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_4_4(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_8_2(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_16_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_8_4(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_32_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 hipError_t launch_team_64_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status);
 
@@ -107,7 +107,7 @@ hipError_t launch_buckets(const int32_t* nact, int64_t ncell, int naq, int32_t* 
 }
 
 hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* ds,
-                            int64_t ncell, double dt, bool mixed, const LaunchCfg& cfg,
+                            int64_t ncell, double dt, int mixed, const LaunchCfg& cfg,
                             unsigned long long* cursors, int num_cus, int nxcd,
                             hipStream_t stream, bool has_status) {
 #define PFCHEM_SHAPE(LL, RR)                                                               \

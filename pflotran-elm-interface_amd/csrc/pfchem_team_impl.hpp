@@ -1287,13 +1287,20 @@ hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int6
   return hipGetLastError();
 }
 
-// FP64 variant, or the mixed variant (FP32 pass, then an FP64 pass over the
-// cells the FP32 pass handed over)
+// mixed = 0: FP64 variant; 1: mixed variant (FP32 pass, then an FP64 pass
+// over the cells the FP32 pass handed over); kTeamRescue64: only the FP64
+// pass over cells an earlier pass handed over (status kEscalate)
 template <int L, int R, bool SD, bool VAR>
 hipError_t launch_variant(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                          double dt, bool mixed, const LaunchCfg& cfg,
+                          double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
                           hipStream_t stream, bool has_status) {
+  if (mixed == kTeamRescue64) {
+    LaunchCfg c64 = cfg;
+    while (c64.block_size > 64 && !team_supports(n, c64, false)) c64.block_size /= 2;
+    return launch_LR<L, R, double, SD, VAR>(n, dn, ds, ncell, dt, c64, cursors, num_cus, nxcd,
+                                            stream, 1);
+  }
   if (!mixed)
     return launch_LR<L, R, double, SD, VAR>(n, dn, ds, ncell, dt, cfg, cursors, num_cus, nxcd,
                                             stream, 0);
@@ -1308,7 +1315,7 @@ hipError_t launch_variant(const DevNet& n, const DevNet* dn, const DevState* ds,
 
 template <int L, int R>
 hipError_t launch_pair(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                       double dt, bool mixed, const LaunchCfg& cfg,
+                       double dt, int mixed, const LaunchCfg& cfg,
                        unsigned long long* cursors, int num_cus, int nxcd,
                        hipStream_t stream, bool has_status) {
   if (n.variable) {

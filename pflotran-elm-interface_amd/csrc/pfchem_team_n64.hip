@@ -5,7 +5,7 @@
 namespace pfchem {
 
 hipError_t launch_team_64_1(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
-                            double dt, bool mixed, const LaunchCfg& cfg,
+                            double dt, int mixed, const LaunchCfg& cfg,
                             unsigned long long* cursors, int num_cus, int nxcd,
                             hipStream_t stream, bool has_status) {
   return launch_pair<64, 1>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus, nxcd, stream,

@@ -56,8 +56,24 @@ def load(path: str = LIB_PATH):
                                  vp]
     L.pfchem_adjoint.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, C.c_int, vp, vp, vp, vp,
                                  vp]
+    L.pfchem_prepare_kernels.argtypes = [vp, vp, C.c_char_p]
     _lib = L
     return L
+
+
+def prepare_kernels(net: ReactionNetwork, opt: SolverOptions,
+                    source_path: Optional[str] = None) -> int:
+    """pfchem_prepare_kernels: generate and compile (no device needed) the
+    network-specialized one-lane kernels of (net, opt).  Returns PFCHEM_OK,
+    or PFCHEM_ERR_UNSUPPORTED when the one-lane kernel does not take the
+    network; raises on a compilation failure."""
+    L = load()
+    n, o = net.to_abi(), opt.to_abi()
+    rc = L.pfchem_prepare_kernels(C.byref(n), C.byref(o),
+                                  source_path.encode() if source_path else None)
+    if rc not in (abi.PFCHEM_OK, abi.PFCHEM_ERR_UNSUPPORTED):
+        raise PfchemError(rc, "pfchem_prepare_kernels")
+    return rc
 
 
 def _stream_ptr(stream):

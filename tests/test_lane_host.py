@@ -1,0 +1,98 @@
+"""The network-specialized one-lane kernel's generated source, compiled for the
+host (one thread plays a one-lane wavefront, tests/lane_host), against the FP64
+oracle on the same seeded cells.  This checks the generated arithmetic, the
+RStep / RReact state machine (reaction.F90:3564-4055), the FP32 hand-over to
+the FP64 pass and the write-out without a GPU; the GPU tests check the device
+build of the same source.
+
+Bar: relative concentration error <= 1e-8 against the oracle (measured:
+calcite 7e-15 FP64 / 2e-11 mixed; 15-primary network 2e-12 / 5e-11), the
+oracle's FP64 relative residual of the answer <= 1e-10 (or the oracle's own),
+and the FP64 pass takes the oracle's Newton iteration counts.
+"""
+import tempfile
+
+import numpy as np
+import pytest
+
+from pfchem import abi, workloads, lib
+from pfchem.network import SolverOptions
+
+import gpu_util as G
+import lane_util as LU
+
+CONC_TOL = 1e-8
+
+
+def _check(wl, ref, rec, got, label, iters_equal):
+    n = wl.state.ncell
+    assert np.all(got["status"][0, :n] == 0), label
+    errs = G.compare(ref, got, n)
+    print(label, errs)
+    for f, e in errs.items():
+        assert e <= CONC_TOL, (label, f, e)
+    rr, bar = G.gate_rel_residual(wl, got, rec, range(n), ref)
+    assert np.all(rr <= bar), label
+    same = np.mean(got["num_iterations"][0, :n] == ref["num_iterations"][0, :n])
+    assert same >= iters_equal, (label, same)
+
+
+@pytest.fixture(scope="module")
+def calcite_host():
+    wl = workloads.calcite_workload(1000)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    exe = LU.build_host(wl.net, wl.opt, tempfile.mkdtemp())
+    return wl, ref, rec, exe
+
+
+def test_calcite_fp64(calcite_host):
+    wl, ref, rec, exe = calcite_host
+    got = LU.run_host(exe, wl.state, wl.dt, f32=False)
+    _check(wl, ref, rec, got, "calcite/lane-fp64", 0.999)
+
+
+def test_calcite_mixed(calcite_host):
+    wl, ref, rec, exe = calcite_host
+    got = LU.run_host(exe, wl.state, wl.dt, f32=True)
+    _check(wl, ref, rec, got, "calcite/lane-mixed", 0.99)
+
+
+def test_random15_both(tmp_path):
+    wl = workloads.random_network_workload(ncell=2048)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    exe = LU.build_host(wl.net, wl.opt, str(tmp_path))
+    _check(wl, ref, rec, LU.run_host(exe, wl.state, wl.dt, f32=False), "random15/lane-fp64",
+           0.995)
+    _check(wl, ref, rec, LU.run_host(exe, wl.state, wl.dt, f32=True), "random15/lane-mixed",
+           0.99)
+
+
+def test_hand_over_without_later_pass(calcite_host):
+    """with no later pass (escalate = 0), a cell whose FP32 Newton does not
+    converge within the iteration limit is cut like the reference's and
+    finishes: a 1-iteration limit forces every cell through the cuts"""
+    import dataclasses
+    wl, ref, rec, _ = calcite_host
+    opt = dataclasses.replace(wl.opt, maximum_reaction_iterations=1, maximum_reaction_cuts=2)
+    wl2 = dataclasses.replace(wl, opt=opt)
+    st = wl.state.subset(np.arange(64))
+    ref2 = st.copy()
+    import oracle_lib as O
+    O.step_batch(wl.net, opt, ref2, wl.dt, 0)
+    exe = LU.build_host(wl.net, opt, tempfile.mkdtemp())
+    got = LU.run_host(exe, st, wl.dt, f32=False, escalate=False)
+    np.testing.assert_array_equal(got["status"][0], ref2["status"][0])
+    np.testing.assert_array_equal(got["num_iterations"][0], ref2["num_iterations"][0])
+    for f in ("pri_molal", "total", "sec_molal", "mnrl_volfrac"):
+        np.testing.assert_allclose(got[f], ref2[f], rtol=1e-12, atol=0)
+
+
+def test_prepare_refuses_other_networks():
+    """the one-lane kernel takes aqueous networks of <= 16 primaries without
+    the sandbox or per-cell active counts; the others report UNSUPPORTED"""
+    wl = workloads.variable_network_workload(ncell=8)
+    assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
+    wl = workloads.somdec_workload(ncell=8)
+    assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
