@@ -8,11 +8,16 @@ synthetic seeded inputs.  A "step" is one pfchem_step over all cells; each
 timed step works on its own HBM-resident copy of the same input state, so
 all K steps solve the same problem back to back.
 
-Reported: the batched mixed-precision kernel (headline value), the plain
-FP64 one-cell-per-thread kernel it is measured against, the batched FP64
-kernel, the roofline of the dominant kernel, the FP64 CPU oracle timed on
-this host's cores (cpu_baseline), and the parity of the GPU answer against
-that oracle (relative concentration error and FP64 relative residual).
+Reported: the batched mixed-precision kernel (headline value, the
+network-specialized one-cell-per-lane kernel), the plain FP64
+one-cell-per-thread kernel it is measured against, the batched FP64 kernel,
+the team kernel of both precisions, the roofline of the dominant kernel, the
+FP64 CPU oracle timed on this host's cores (cpu_baseline), and the parity of
+the GPU answer against that oracle (relative concentration error and FP64
+relative residual).  The headline value is the mixed kernel's only when its
+answer meets the bar (<= 1e-8 relative concentration error against the
+oracle, relative residual <= 1e-10 or the oracle's own); otherwise it is the
+batched FP64 kernel's, and the line says so.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cells N]
 """
@@ -113,9 +118,19 @@ def main():
     plain_cps = args.plain_steps * n / (plain_ms * 1e-3)
     log("[bench] plain_fp64    %.2f ms/step  %.3e cells/s" % (plain_ms / args.plain_steps, plain_cps))
     # --- batched FP64 ---
-    b64_state, b64_ms, _, _ = run(abi.KERNEL_BATCHED_FP64, max(2, args.steps // 2), 1)
+    b64_state, b64_ms, b64_per, _ = run(abi.KERNEL_BATCHED_FP64, max(2, args.steps // 2), 1)
     b64_cps = max(2, args.steps // 2) * n / (b64_ms * 1e-3)
     log("[bench] batched_fp64  %.2f ms/step  %.3e cells/s" % (b64_ms / max(2, args.steps // 2), b64_cps))
+    # --- the team kernel (16 lanes per cell) of both precisions ---
+    team = {}
+    for v, name in ((abi.KERNEL_BATCHED_FP64, "fp64"), (abi.KERNEL_BATCHED_MIXED, "mixed")):
+        saved = ctx.get_launch(v)
+        ctx.set_launch(v, 16, 256, 4, 1)
+        _, t_ms, _, _ = run(v, 2, 1)
+        ctx.set_launch(v, saved["lanes_per_cell"], saved["block_size"], saved["blocks_per_cu"],
+                       saved["rows_per_lane"])
+        team[name + "_cells_per_s"] = 2 * n / (t_ms * 1e-3)
+        log("[bench] team16 %-6s %.2f ms/step  %.3e cells/s" % (name, t_ms / 2, team[name + "_cells_per_s"]))
     # --- batched mixed precision (headline) ---
     mix_state, mix_ms, mix_per, wall = run(abi.KERNEL_BATCHED_MIXED, args.steps, args.warmup)
     ms_per_step = mix_ms / args.steps
@@ -226,10 +241,16 @@ def main():
                 b = g[f][:, :n][:, ok]
                 if a.size == 0:
                     continue
-                # totals and rates can cancel to ~0: relative to 1e-12 of the
-                # largest magnitude of that species
-                floor = (1e-12 * np.abs(a).max(axis=1, keepdims=True)
-                         if f in ("total", "mnrl_rate") else 0.0)
+                # totals can cancel to ~0: relative to 1e-12 of the largest
+                # magnitude of that species; rates Im = -A k (1 - QK) vanish at
+                # equilibrium: relative to max(|Im|, k A), the rate at unit
+                # affinity
+                floor = 0.0
+                if f == "total":
+                    floor = 1e-12 * np.abs(a).max(axis=1, keepdims=True)
+                elif f == "mnrl_rate":
+                    ka = net.kinmnrl_rate_constant[:, None] * host["mnrl_area"][:, :n][:, ok]
+                    floor = np.maximum(ka, 1e-300)
                 d = np.abs(b - a)
                 with np.errstate(divide="ignore", invalid="ignore"):
                     r = np.where(d == 0, 0.0, d / np.maximum(np.abs(a), floor))
@@ -241,7 +262,8 @@ def main():
             return {label + "_failed_cells": int((gs != 0).sum()),
                     label + "_max_rel_err": errs,
                     label + "_max_rel_conc_err": max(errs.get("pri_molal", 0.0),
-                                                     errs.get("sec_molal", 0.0)),
+                                                     errs.get("sec_molal", 0.0),
+                                                     errs.get("total", 0.0)),
                     label + "_max_rel_residual": float(rr.max()),
                     label + "_cells_rel_residual_le_1e-10": float(np.mean(rr <= 1e-10)),
                     label + "_max_rel_residual_over_oracle_floor": float((rr / floor).max()),
@@ -253,7 +275,21 @@ def main():
         parity["oracle_max_rel_residual"] = float(fr_ref[(ref["status"][0, :n] == 0)].max())
         log("[bench] oracle %.2f s on %d threads; parity: %s" % (t_all, nthr, json.dumps(parity)))
 
-    # --- roofline of the dominant kernel (team_step_kernel, mixed) ---
+    # --- the bar: the mixed answer's concentrations within 1e-8 of the
+    # oracle's and its relative residual <= 1e-10 (or the oracle's own) ---
+    def gate(label):
+        return (parity.get(label + "_failed_cells", 1) == 0 and
+                parity.get(label + "_max_rel_conc_err", 1.0) <= 1e-8 and
+                parity.get(label + "_max_rel_residual_over_oracle_floor", 2.0) <= 1.0)
+    headline = "batched_mixed"
+    if parity and not gate("mixed"):
+        headline = "batched_fp64"
+        value = b64_cps
+        ms_per_step = b64_ms / max(2, args.steps // 2)
+        mix_per = b64_per
+        log("[bench] the mixed answer misses the bar: headline is the batched FP64 kernel")
+
+    # --- roofline of the dominant kernel (the one-lane kernel's pass) ---
     active = host["mnrl_area"][:, :n] > 0
     f_it = flops.per_iteration_flops(net, opt, active)
     flop_alg = float((iters_ref * f_it).sum())
@@ -261,15 +297,16 @@ def main():
     achieved = flop_alg / (k_ms * 1e-3) / 1e12
     bytes_alg = flops.state_bytes_per_cell(net) * n
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_team_step_random15.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_lane_step_random15.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     roofline = {"bound": "valu", "achieved": achieved, "peak": MI355X_FP64_VALU_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / MI355X_FP64_VALU_TFLOPS,
                 "traffic": traffic,
-                "kernel": "team_step_kernel<16,1,float> (FP32 pass; the FP64 rescue pass "
-                          "over escalated cells is inside the same step)",
+                "kernel": ("pfchem_lane_step_%s (network-specialized, one cell per lane); the "
+                           "time is the whole step, including the passes over handed-over "
+                           "cells" % ("f32" if headline == "batched_mixed" else "f64")),
                 "algorithmic_flop_per_launch": flop_alg,
                 "algorithmic_bytes_per_launch": bytes_alg,
                 "hbm_gbs_algorithmic": bytes_alg / (k_ms * 1e-3) / 1e9,
@@ -279,11 +316,13 @@ def main():
         "metric": METRIC, "value": value, "unit": "cells/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 residual/logK/update + f32 Jacobian/LU", "data": "synthetic",
+        "dtype": ("f64 residual, log K sums, exponentials, totals, update and tests; f32 "
+                  "off-diagonal Jacobian assembly and L D L^T" if headline == "batched_mixed"
+                  else "f64"), "data": "synthetic",
         "config": {"workload": "random aqueous network: 15 primaries, 60 complexes, "
                                "5 kinetic minerals (0-5 active per cell)",
                    "cells": n, "dt_s": wl.dt, "seed": wl.seed,
-                   "kernel": "batched_mixed", "launch": launch,
+                   "kernel": headline, "launch": launch,
                    "tolerances": {"max_rel_residual_tolerance": opt.max_rel_residual_tolerance,
                                   "max_relative_change_tolerance": opt.max_relative_change_tolerance,
                                   "max_residual_tolerance": opt.max_residual_tolerance,
@@ -292,6 +331,7 @@ def main():
         "cpu_baseline": cpu,
         "plain_fp64_cells_per_s": plain_cps,
         "batched_fp64_cells_per_s": b64_cps,
+        "team16_cells_per_s": team,
         "speedup_vs_plain_fp64": value / plain_cps,
         "girt_residual_jacobian": girt,
         "constraint_speciation": equil,

@@ -232,6 +232,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
   int nconst = 0, ncuts = 0, nsub = 0, nit_total = 0, nit = 0, status = 0;
   int attempts = 0, last_why = 0;
   bool restore = false;    // outputs of a failed RStep: totals at RReact start
+  bool kin = false;        // write-out: the last sub-step's kinetic update is due
   unsigned small = 0u;     // bit i: component i started at or below 1e-40
 #pragma unroll
   for (int i = 0; i < N; ++i) x[i] = 1.0;
@@ -349,6 +350,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
       tdt = A.dt;
       nconst = ncuts = nsub = nit_total = status = attempts = last_why = 0;
       restore = false;
+      kin = false;
       nit = -1;
       mode = 1;
     }
@@ -541,6 +543,18 @@ __device__ __forceinline__ void lane_step(const Args& A) {
     if (wout) {
       // ---------------- write the cell back ----------------
       const bool ok = status == 0;
+      if (kin) {
+        // RUpdateKineticState of the last sub-step (reaction_mineral.F90:
+        // 1456-1525): rates at the final state, volume fractions of the
+        // sub-step's start
+        sfor<0, NM>([&](auto M) {
+          constexpr int m = decltype(M)::value;
+          double v = PF_VF(m) + rt[m] * Net::MNRL_VM[m] * tdt;
+          if (v < 0.0) v = 0.0;
+          PF_VF(m) = v;
+          PF_RLAST(m) = rt[m];
+        });
+      }
       sfor<0, NAQ>([&](auto Q) {
         constexpr int i = decltype(Q)::value;
         A.pri_molal[i * ld + c] = x[i];
@@ -581,11 +595,20 @@ __device__ __forceinline__ void lane_step(const Args& A) {
     if (nit == 1) r0 = nr;
     // 0 continue, 1 converged, 2 max iterations, 3 hand over
     int outcome = 0;
+    // FP32: when the convergence test first passes on the last sub-step, one
+    // more FP32-preconditioned correction of the FP64 residual is applied
+    // before the cell exits (the write-out evaluates at the corrected
+    // state): the exit test bounds the residual, not the error, and the
+    // linear convergence of the FP32-preconditioned iteration would leave
+    // up to about 1e-9 of relative error at the exit
+    bool polish = false;
     if (nit > Net::MAX_IT) {
       outcome = 2;
     } else if (mx < Net::TOL_ABS || nr / r0 < Net::TOL_REL) {
-      outcome = 1;
-    } else {
+      if (F32 && cum + tdt >= target) polish = true;
+      else outcome = 1;
+    }
+    if (outcome == 0) {
       // ---------------- linear solve (RSolve) ----------------
       // aqueous block: A = D^-1/2 M D^-1/2, L D L^T, A y = D^-1/2 res / jc
       double s[NA1];
@@ -705,10 +728,15 @@ __device__ __forceinline__ void lane_step(const Args& A) {
           const double q = fabs((latest[i] - x[i]) * rcp_nr(x[i]));
           if (q == q) rc = fmax(rc, q);
         }
-        // an FP32 step is only as accurate as the FP32 factorization: a
-        // small-step exit needs the FP64 residual to confirm it
-        const bool step_exit_ok = !F32 || nr <= Net::TOL_REL * r0;
-        if (rc >= 0.0 && rc < Net::TOL_CHANGE && step_exit_ok) {
+        // The small-step exit holds for the FP32 factorization too: the
+        // scaled matrix has unit diagonal and |A_ij| <= 1 (positive
+        // semi-definite terms), so |A| <= n and a step below the tolerance
+        // certifies a scaled FP64 residual |b| <= n |y| of the same size.
+        if (polish) {
+#pragma unroll
+          for (int i = 0; i < N; ++i) x[i] = latest[i];
+          outcome = 1;
+        } else if (rc >= 0.0 && rc < Net::TOL_CHANGE) {
           outcome = 1;
         } else {
 #pragma unroll
@@ -719,13 +747,17 @@ __device__ __forceinline__ void lane_step(const Args& A) {
 
     // ---------------- RReact / RStep transitions ----------------
     if (outcome == 1) {
-      // RUpdateKineticState (reaction_mineral.F90:1456-1525)
-      sfor<0, NM>([&](auto M) {
-        constexpr int m = decltype(M)::value;
-        double v = PF_VF(m) + rt[m] * Net::MNRL_VM[m] * tdt;
-        if (v < 0.0) v = 0.0;
-        PF_VF(m) = v;
-      });
+      // RUpdateKineticState (reaction_mineral.F90:1456-1525); the last
+      // sub-step's is done by the write-out evaluation at the final state
+      const bool last = cum + tdt >= target;
+      if (!last) {
+        sfor<0, NM>([&](auto M) {
+          constexpr int m = decltype(M)::value;
+          double v = PF_VF(m) + rt[m] * Net::MNRL_VM[m] * tdt;
+          if (v < 0.0) v = 0.0;
+          PF_VF(m) = v;
+        });
+      }
       nit_total += nit;
       cum += tdt;
       ++nsub;
@@ -738,14 +770,14 @@ __device__ __forceinline__ void lane_step(const Args& A) {
       for (int i = NAQ; i < N; ++i) PF_TINIT(i) = x[i];
 #pragma unroll
       for (int i = 0; i < N; ++i) PF_GUESS(i) = x[i];
-      if (nconst >= 4) {
-        --ncuts;
-        tdt = fmin(2.0 * tdt, target - cum);
-      }
       nit = -1;
       if (cum >= target) {
         status = 0;
-        mode = 2;
+        kin = true;
+        mode = 2;  // tdt stays the last sub-step's (the kinetic update uses it)
+      } else if (nconst >= 4) {
+        --ncuts;
+        tdt = fmin(2.0 * tdt, target - cum);
       }
     } else if (outcome == 3 || (outcome == 2 && F32 && A.escalate)) {
       // hand the untouched cell to the next pass

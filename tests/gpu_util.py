@@ -44,17 +44,33 @@ def rel_err(a, b, floor=0.0):
     return r
 
 
-def compare(ref, got, ncell, fields=FIELDS_F64, floor_frac=1e-12):
+def rate_scale(net, state, ncell):
+    """per kinetic mineral: k A, the TST rate at unit affinity (largest area
+    over the cells), mol/m^3/s"""
+    k = np.asarray(net.kinmnrl_rate_constant, dtype=np.float64)
+    a = state["mnrl_area"][:, :ncell].max(axis=1) if ncell else np.zeros(k.size)
+    return (k * a)[:, None]
+
+
+def compare(ref, got, ncell, fields=FIELDS_F64, floor_frac=1e-12, net=None):
     """max relative error per field over the first ncell cells.  Totals can be
     near zero, so their relative error uses a floor of floor_frac times the
-    largest magnitude of that species over the cells."""
+    largest magnitude of that species over the cells.  Mineral rates
+    Im = -A k (1 - QK) vanish at equilibrium, where their relative error is
+    the concentration error amplified by QK / |1 - QK|: with the network
+    given, they are compared relative to max(|Im|, k A) (k A: the rate at
+    unit affinity, |1 - QK| = 1), else with the totals' floor."""
     out = {}
     for f in fields:
         a = ref[f][:, :ncell]
         b = got[f][:, :ncell]
         if a.size == 0:
             continue
-        floor = floor_frac * np.abs(a).max(axis=1, keepdims=True) if f in ("total", "mnrl_rate") else 0.0
+        floor = 0.0
+        if f == "total" or (f == "mnrl_rate" and net is None):
+            floor = floor_frac * np.abs(a).max(axis=1, keepdims=True)
+        elif f == "mnrl_rate":
+            floor = rate_scale(net, ref, ncell)
         out[f] = float(rel_err(b, a, floor).max())
     return out
 

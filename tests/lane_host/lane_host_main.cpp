@@ -54,7 +54,7 @@ int main(int argc, char** argv) {
   a.rescue = 0;
   if (f32) {
     pfchem_lane_step_f32(a);
-    if (a.escalate) {
+    if (a.escalate && !std::getenv("PFCHEM_HOST_NO_RESCUE")) {
       cursors[0] = 0;
       a.rescue = 1;
       pfchem_lane_step_f64(a);

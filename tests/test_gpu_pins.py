@@ -1,0 +1,95 @@
+"""GPU parity on the paths the oracle pin tests cover (tests/test_oracle_pins.py):
+immobile species with first-order decay (RImmobileDecay,
+reaction_immobile.F90:244-294), temperature-dependent log K with an Arrhenius
+rate constant (ReactionInterpolateLogK, reaction_aux.F90:1285-1312), the
+kinetic-mineral rate options (reaction_mineral.F90:729-908), and
+RActivityCoefficients LAG through pfchem_update_activity
+(reaction.F90:4555-4611).  Every kernel variant and both batched kernels (the
+one-lane kernel, the default, and a 16x1 team) run through the C ABI against
+the FP64 oracle on the same seeded cells.
+
+Bar: relative concentration error <= 1e-8 against the oracle, FP64 relative
+residual of the answer <= 1e-10 (or twice the oracle's own where rounding
+keeps it above), activity coefficients <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from pfchem import abi, workloads
+from pfchem.lib import Context, DeviceState
+
+import gpu_util as G
+import pin_util as PU
+
+pytestmark = pytest.mark.gpu
+
+CONC_TOL = 1e-8
+# (variant, lanes per cell: None = the context's default)
+KERNELS = [(abi.KERNEL_PLAIN_FP64, None), (abi.KERNEL_BATCHED_FP64, None),
+           (abi.KERNEL_BATCHED_MIXED, None), (abi.KERNEL_BATCHED_FP64, 16),
+           (abi.KERNEL_BATCHED_MIXED, 16)]
+IDS = ["plain", "lane-fp64", "lane-mixed", "team16-fp64", "team16-mixed"]
+
+
+def _parity(wl, variant, lanes, label):
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    print("case", label, flush=True)
+    got, _ = G.run_gpu(wl, variant, lanes=lanes)
+    n = wl.state.ncell
+    assert np.all(got["status"][0, :n] == 0), label
+    fields = G.FIELDS_F64 if wl.net.nimmobile else tuple(f for f in G.FIELDS_F64
+                                                         if f != "immobile")
+    errs = G.compare(ref, got, n, fields, net=wl.net)
+    print(label, errs, flush=True)
+    for f, e in errs.items():
+        assert e <= CONC_TOL, (label, f, e)
+    rr, bar = G.gate_rel_residual(wl, got, rec, range(n), ref)
+    assert rr.size > 0.8 * n and np.all(rr <= bar), label
+
+
+@pytest.mark.parametrize("variant,lanes", KERNELS, ids=IDS)
+def test_decay(variant, lanes):
+    wl, _ = PU.decay_workload(512)
+    _parity(wl, variant, lanes, "decay/" + IDS[KERNELS.index((variant, lanes))])
+
+
+@pytest.mark.parametrize("variant,lanes", KERNELS, ids=IDS)
+def test_nonisothermal(variant, lanes):
+    wl = PU.nonisothermal_workload(512)
+    _parity(wl, variant, lanes, "nonisothermal/" + IDS[KERNELS.index((variant, lanes))])
+
+
+OPTION_SETS = [dict(kinmnrl_Temkin_const=2.0, kinmnrl_min_scale_factor=1.5),
+               dict(kinmnrl_affinity_power=1.7),
+               dict(kinmnrl_rate_limiter=3.0, kinmnrl_affinity_threshold=2.0,
+                    kinmnrl_irreversible=1)]
+
+
+@pytest.mark.parametrize("opts", OPTION_SETS, ids=["temkin+scale", "power", "limiter+cutoffs"])
+@pytest.mark.parametrize("variant,lanes", KERNELS, ids=IDS)
+def test_mineral_options(variant, lanes, opts):
+    wl = PU.mineral_options_workload(512, **opts)
+    _parity(wl, variant, lanes, "options/" + IDS[KERNELS.index((variant, lanes))])
+
+
+def test_update_activity_lag():
+    """pfchem_update_activity (LAG) against the oracle's
+    RActivityCoefficients on the 15-primary network's cells"""
+    import torch
+    wl = workloads.random_network_workload(ncell=4096)
+    st = wl.state.copy()
+    # move the state away from the coefficients it was built with
+    st["pri_molal"][:, :] *= 1.3
+    ref = st.copy()
+    O.update_activity_batch(wl.net, wl.opt, ref)
+    ctx = Context(wl.net, wl.opt)
+    ds = DeviceState(st)
+    ctx.update_activity(ds)
+    torch.cuda.synchronize()
+    got = ds.to_host()
+    for f in ("pri_act_coef", "sec_act_coef"):
+        e = float(G.rel_err(got[f], ref[f]).max())
+        print("update_activity", f, e, flush=True)
+        assert e <= 1e-12, (f, e)

@@ -41,7 +41,7 @@ def _check(wl, ref, rec, got, label):
     n = wl.state.ncell
     print("case", label, flush=True)
     assert np.all(got["status"][0, :n] == 0), "%s: failed cells" % label
-    errs = G.compare(ref, got, n)
+    errs = G.compare(ref, got, n, net=wl.net)
     print(label, errs, flush=True)
     for f, e in errs.items():
         assert e <= CONC_TOL, "%s: %s rel err %.3e > %.1e" % (label, f, e, CONC_TOL)
@@ -151,6 +151,6 @@ def test_ld_padding_and_partial_range(randnet):
     ctx.step(ds, padded.dt, abi.KERNEL_BATCHED_MIXED, ncell=half)
     torch.cuda.synchronize()
     got = ds.to_host()
-    errs = G.compare(ref, got, half)
+    errs = G.compare(ref, got, half, net=wl.net)
     assert max(errs.values()) <= CONC_TOL
     assert np.array_equal(got["pri_molal"][:, half:], padded.state["pri_molal"][:, half:])

@@ -8,7 +8,8 @@ build of the same source.
 Bar: relative concentration error <= 1e-8 against the oracle (measured:
 calcite 7e-15 FP64 / 2e-11 mixed; 15-primary network 2e-12 / 5e-11), the
 oracle's FP64 relative residual of the answer <= 1e-10 (or the oracle's own),
-and the FP64 pass takes the oracle's Newton iteration counts.
+and the FP64 pass takes the oracle's Newton iteration counts (the mixed pass
+converges linearly near the root, so its counts may differ by one).
 """
 import tempfile
 
@@ -27,7 +28,7 @@ CONC_TOL = 1e-8
 def _check(wl, ref, rec, got, label, iters_equal):
     n = wl.state.ncell
     assert np.all(got["status"][0, :n] == 0), label
-    errs = G.compare(ref, got, n)
+    errs = G.compare(ref, got, n, net=wl.net)
     print(label, errs)
     for f, e in errs.items():
         assert e <= CONC_TOL, (label, f, e)
@@ -55,7 +56,8 @@ def test_calcite_fp64(calcite_host):
 def test_calcite_mixed(calcite_host):
     wl, ref, rec, exe = calcite_host
     got = LU.run_host(exe, wl.state, wl.dt, f32=True)
-    _check(wl, ref, rec, got, "calcite/lane-mixed", 0.99)
+    # the FP32-preconditioned iteration converges linearly: counts may differ
+    _check(wl, ref, rec, got, "calcite/lane-mixed", 0.9)
 
 
 def test_random15_both(tmp_path):
@@ -66,7 +68,7 @@ def test_random15_both(tmp_path):
     _check(wl, ref, rec, LU.run_host(exe, wl.state, wl.dt, f32=False), "random15/lane-fp64",
            0.995)
     _check(wl, ref, rec, LU.run_host(exe, wl.state, wl.dt, f32=True), "random15/lane-mixed",
-           0.99)
+           0.9)
 
 
 def test_hand_over_without_later_pass(calcite_host):
@@ -96,3 +98,40 @@ def test_prepare_refuses_other_networks():
     assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
     wl = workloads.somdec_workload(ncell=8)
     assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
+
+
+# the paths the pin tests cover: decay, temperature-dependent log K with
+# Arrhenius, and the kinetic-mineral rate options
+import pin_util as PU
+
+
+def _run_pin_case(wl, label, iters_equal=0.99):
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    exe = LU.build_host(wl.net, wl.opt, tempfile.mkdtemp())
+    for f32 in (False, True):
+        got = LU.run_host(exe, wl.state, wl.dt, f32=f32)
+        _check(wl, ref, rec, got, label + ("/mixed" if f32 else "/fp64"),
+               iters_equal if not f32 else 0.9)
+        for f in ("immobile",):
+            if wl.net.nimmobile:
+                e = float(G.rel_err(got[f], ref[f]).max())
+                assert e <= CONC_TOL, (label, f, e)
+
+
+def test_decay_network():
+    wl, _ = PU.decay_workload(256)
+    _run_pin_case(wl, "decay")
+
+
+def test_nonisothermal_network():
+    _run_pin_case(PU.nonisothermal_workload(256), "non-isothermal")
+
+
+@pytest.mark.parametrize("opts", [
+    dict(kinmnrl_Temkin_const=2.0, kinmnrl_min_scale_factor=1.5),
+    dict(kinmnrl_affinity_power=1.7),
+    dict(kinmnrl_rate_limiter=3.0, kinmnrl_affinity_threshold=2.0, kinmnrl_irreversible=1),
+], ids=["temkin+scale", "power", "limiter+threshold+irreversible"])
+def test_mineral_option_networks(opts):
+    _run_pin_case(PU.mineral_options_workload(256, **opts), "options")

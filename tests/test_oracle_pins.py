@@ -27,7 +27,7 @@ import pytest
 
 import oracle_lib as O
 from pfchem import abi, database, workloads
-from pfchem.network import CellState
+from pin_util import decay_workload
 
 GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
 
@@ -35,21 +35,6 @@ GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "go
 # ---------------------------------------------------------------------------
 # immobile decay
 # ---------------------------------------------------------------------------
-def decay_workload(ncell=200, k=(2.0e-5, 3.0e-5)):
-    base = workloads.calcite_workload(ncell)
-    net = dataclasses.replace(base.net, immobile_names=["B1", "B2"],
-                              decayspecid=np.array([0, 0], dtype=np.int32),
-                              decay_rate_constant=np.array(k))
-    st = CellState(net, ncell)
-    for name, a in base.state.arrays.items():
-        if a.shape[0]:
-            st.arrays[name][...] = a
-    rng = np.random.default_rng(7)
-    st["immobile"][0, :] = 10.0 ** rng.uniform(-4, 0, ncell)
-    st["immobile"][1, :] = 10.0 ** rng.uniform(-4, 0, ncell)
-    return dataclasses.replace(base, net=net, state=st), base
-
-
 def test_decay_closed_form():
     wl, base = decay_workload()
     ref = wl.state.copy()
