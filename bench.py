@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--only", choices=["plain", "fp64", "mixed", "team-fp64", "team-mixed"],
+                    help="time only this kernel of the step (profiling runs) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (calibrates the "
                          "HBM byte counters of a profiler) and exit")
@@ -112,6 +114,17 @@ def main():
         total_ms = ev0.elapsed_time(ev1)
         per = [a.elapsed_time(b) for a, b in kev]
         return work[-1], total_ms, per, wall
+
+    if args.only:
+        variant = {"plain": abi.KERNEL_PLAIN_FP64, "fp64": abi.KERNEL_BATCHED_FP64,
+                   "mixed": abi.KERNEL_BATCHED_MIXED, "team-fp64": abi.KERNEL_BATCHED_FP64,
+                   "team-mixed": abi.KERNEL_BATCHED_MIXED}[args.only]
+        if args.only.startswith("team"):
+            ctx.set_launch(variant, 16, 256, 4, 1)
+        _, t_ms, per, _ = run(variant, args.steps, args.warmup)
+        print(json.dumps({"only": args.only, "steps": args.steps, "ms_per_step": t_ms / args.steps,
+                          "step_ms": per, "cells": n}), flush=True)
+        return 0
 
     # --- plain FP64 one-cell-per-thread kernel (the reference point) ---
     plain_state, plain_ms, plain_per, _ = run(abi.KERNEL_PLAIN_FP64, args.plain_steps, 1)

@@ -126,8 +126,22 @@ int lane_supported(const pfchem_network* net, const pfchem_options* opt, std::st
   if (naq + nimm > 32) { why = "more than 32 unknowns"; return 0; }
   if (net->neqcplx > kLaneMaxCplx) { why = "too many complexes"; return 0; }
   if (net->nkinmnrl > kLaneMaxMnrl) { why = "too many kinetic minerals"; return 0; }
-  // per-lane LDS slots: 2 (naq + nimm) + naq + 3 nkinmnrl doubles x 256 lanes
-  if (2 * (naq + nimm) + naq + 3 * net->nkinmnrl > kLaneMaxLdsFields) {
+  // per-lane LDS slots: 2 (naq + nimm) + naq + 3 nkinmnrl doubles plus one
+  // per activity-coefficient class of the complexes, x 256 lanes
+  int ncls = 0;
+  {
+    std::vector<std::pair<double, double>> cls;
+    for (int k = 0; k < net->neqcplx; ++k) {
+      const double z = net->eqcplx_Z ? net->eqcplx_Z[k] : 0.0;
+      if (!(std::fabs(z) > 1.0e-10)) continue;
+      const std::pair<double, double> key(z * z, net->eqcplx_a0 ? net->eqcplx_a0[k] : 0.0);
+      bool seen = false;
+      for (const auto& c : cls) seen = seen || c == key;
+      if (!seen) cls.push_back(key);
+    }
+    ncls = (int)cls.size();
+  }
+  if (2 * (naq + nimm) + naq + 3 * net->nkinmnrl + ncls > kLaneMaxLdsFields) {
     why = "per-cell state exceeds the LDS slots";
     return 0;
   }
@@ -150,8 +164,8 @@ int lane_supported(const pfchem_network* net, const pfchem_options* opt, std::st
 std::string lane_source(const pfchem_network* net, const pfchem_options* opt) {
   const int naq = net->naqcomp, nimm = net->nimmobile, nc = net->neqcplx, nm = net->nkinmnrl;
   Gen g;
+  // (hiprtc provides the HIP device runtime without an include)
   g.line("// generated: one reaction network and its solver options as constants");
-  g.line("#include <hip/hip_runtime.h>");
   g.line("struct Net {");
   g.i("NAQ", naq);
   g.i("NIMM", nimm);
@@ -203,6 +217,30 @@ std::string lane_source(const pfchem_network* net, const pfchem_options* opt) {
   g.dmat("CPLX_COEF", coef(net->eqcplx_logKcoef, nc), 5);
   g.dvec("CPLX_Z", vec(net->eqcplx_Z, nc, 0.0));
   g.dvec("CPLX_A0", vec(net->eqcplx_a0, nc, 0.0));
+  // activity-coefficient classes of the complexes: one per distinct
+  // (Z^2, a0) of the charged ones (Debye-Hueckel gives them equal
+  // coefficients), -1 for neutral ones (coefficient 1)
+  {
+    std::vector<std::pair<double, double>> cls;
+    std::vector<int> id(nc, -1), first(nc, 0);
+    for (int k = 0; k < nc; ++k) {
+      const double z = net->eqcplx_Z[k];
+      if (!(std::fabs(z) > 1.0e-10)) continue;
+      const std::pair<double, double> key(z * z, net->eqcplx_a0[k]);
+      int q = -1;
+      for (size_t j = 0; j < cls.size(); ++j)
+        if (cls[j] == key) q = (int)j;
+      if (q < 0) {
+        q = (int)cls.size();
+        cls.push_back(key);
+        first[k] = 1;
+      }
+      id[k] = q;
+    }
+    g.i("NGCLS", (long)cls.size());
+    g.ivec("CPLX_GCLS", id);
+    g.ivec("CPLX_GFIRST", first);
+  }
   // kinetic minerals
   std::vector<int> mn;
   std::vector<std::vector<int>> mid;

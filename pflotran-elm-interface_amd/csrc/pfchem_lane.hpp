@@ -96,6 +96,12 @@ __host__ __device__ constexpr int tri(int i, int j) {
   return i > j ? i * (i - 1) / 2 + j : j * (j - 1) / 2 + i;
 }
 
+// FP64 exp / log kept out of line: the network's complexes call them some
+// hundred times per Newton iteration, and one copy keeps the unrolled loop
+// inside the instruction cache
+__device__ __noinline__ double pf_exp(double v) { return exp(v); }
+__device__ __noinline__ double pf_log(double v) { return log(v); }
+
 // 1/g by the hardware reciprocal estimate refined with two Newton-Raphson
 // steps r <- r + r (1 - g r) (within about one ulp of the quotient; g > 0,
 // normal)
@@ -114,7 +120,7 @@ __device__ __forceinline__ double interp_logk(const double (&c)[5], double lnT, 
 
 // Debye-Hueckel activity coefficient (reaction.F90:4586-4592)
 __device__ __forceinline__ double dh_gamma(double z, double a0, double sI, double I) {
-  return exp((-z * z * sI * Net::DH_A / (1.0 + a0 * Net::DH_B * sI) + Net::DH_BDOT * I) *
+  return pf_exp((-z * z * sI * Net::DH_A / (1.0 + a0 * Net::DH_B * sI) + Net::DH_BDOT * I) *
              kLogToLn);
 }
 
@@ -167,7 +173,7 @@ __device__ __forceinline__ MRate mineral_rate(double lnQK, double QK, double vf,
   double kr = Net::MNRL_K[M];
   if constexpr (Net::HAS_EA) {
     if constexpr (Net::MNRL_EA[M] > 0.0)
-      kr = kr * exp(Net::MNRL_EA[M] / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
+      kr = kr * pf_exp(Net::MNRL_EA[M] / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
   }
   double Im_const = -area;
   if constexpr (mscale) Im_const = Im_const / Net::MNRL_MINSCALE[M];
@@ -198,7 +204,11 @@ __device__ __forceinline__ void lane_step(const Args& A) {
   constexpr int NT = NAQ > 1 ? NAQ * (NAQ - 1) / 2 : 1;
   constexpr bool F32 = sizeof(JT) == 4;
   constexpr bool NEWTON_ACT = Net::ACT_FREQ == kFreqNewtonIter && NAQ > 0;
-  constexpr int kLdsFields = 2 * N + NAQ + 3 * NM > 0 ? 2 * N + NAQ + 3 * NM : 1;
+  // per-class inverse activity coefficients of the complexes (activity
+  // coefficients fixed during the step)
+  constexpr int NG = NEWTON_ACT ? 0 : Net::NGCLS;
+  constexpr int NG1 = NG > 0 ? NG : 1;
+  constexpr int kLdsFields = 2 * N + NAQ + 3 * NM + NG > 0 ? 2 * N + NAQ + 3 * NM + NG : 1;
   // pivot tolerance of the unit-diagonal factorization
   constexpr double kPivTol = F32 ? 1.0e-6 : 1.0e-13;
   const i64 ld = A.ld;
@@ -221,6 +231,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
 #define PF_VF(m) lsm[(2 * N + NAQ + (m)) * 256 + tid]
 #define PF_AREA(m) lsm[(2 * N + NAQ + NM + (m)) * 256 + tid]
 #define PF_RLAST(m) lsm[(2 * N + NAQ + 2 * NM + (m)) * 256 + tid]
+#define PF_GINV(q) lsm[(2 * N + NAQ + 3 * NM + (q)) * 256 + tid]
   bool exhausted = false;
   i64 cell = -1;
   int mode = 0;  // 0: no cell, 1: Newton iteration, 2: write-out evaluation
@@ -291,7 +302,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
       vol = A.volume[c];
       temp = A.temperature ? A.temperature[c] : 25.0;
       tk = temp + 273.15;
-      lnT = log(tk);
+      lnT = pf_log(tk);
       ln_aw = (Net::ACT_H2O && A.ln_act_h2o) ? A.ln_act_h2o[c] : 0.0;
       dkl = den * 1.0e-3;
       psv = por * sat * 1000.0 * vol;
@@ -300,7 +311,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
         constexpr int i = decltype(I)::value;
         x[i] = A.pri_molal[i * ld + c];
         tin[i] = A.total[i * ld + c];
-        PF_LNGAM(i) = log(A.pri_act_coef[i * ld + c]);
+        PF_LNGAM(i) = pf_log(A.pri_act_coef[i * ld + c]);
       });
       sfor<0, NIMM>([&](auto I) {
         constexpr int i = decltype(I)::value;
@@ -315,6 +326,33 @@ __device__ __forceinline__ void lane_step(const Args& A) {
           sec_I += mk * Net::CPLX_Z[k] * Net::CPLX_Z[k];
           sec_sum += mk;
         });
+      }
+      if constexpr (!NEWTON_ACT) {
+        // The Debye-Hueckel law gives complexes of equal charge magnitude
+        // and ion size equal coefficients (and neutral ones exactly 1), so
+        // each (Z^2, a0) class keeps one inverse coefficient for the whole
+        // step.  A cell whose input coefficients break that structure is
+        // handed to the team kernel, which reads them per complex.
+        bool cons = true;
+        double g0[NG1];
+        sfor<0, NC>([&](auto K) {
+          constexpr int k = decltype(K)::value;
+          constexpr int q = Net::CPLX_GCLS[k];
+          const double g = A.sec_act_coef[k * ld + c];
+          if constexpr (q < 0) {
+            cons = cons && g == 1.0;
+          } else if constexpr (Net::CPLX_GFIRST[k]) {
+            g0[q] = g;
+            PF_GINV(q) = rcp_nr(g);
+          } else {
+            cons = cons && g == g0[q];
+          }
+        });
+        if (!cons && A.escalate) {
+          A.status[c] = kEscalate;
+          cell = -1;
+          continue;
+        }
       }
       sfor<0, NM>([&](auto M) {
         constexpr int m = decltype(M)::value;
@@ -393,13 +431,13 @@ __device__ __forceinline__ void lane_step(const Args& A) {
         sfor<0, NAQ>([&](auto Q) {
           constexpr int i = decltype(Q)::value;
           if constexpr (Net::ZP[i] > 1.0e-10 || Net::ZP[i] < -1.0e-10)
-            PF_LNGAM(i) = log(dh_gamma(Net::ZP[i], Net::A0P[i], sI, I));
+            PF_LNGAM(i) = pf_log(dh_gamma(Net::ZP[i], Net::A0P[i], sI, I));
           else
             PF_LNGAM(i) = 0.0;
         });
         if constexpr (Net::ACT_H2O) {
           const double a = 1.0 - 0.017 * (sp + sec_sum);
-          ln_aw = a > 0.0 ? log(a) : 0.0;
+          ln_aw = a > 0.0 ? pf_log(a) : 0.0;
         }
         I_last = I;
       } else {
@@ -411,7 +449,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
     JT S[NT];
     sfor<0, NAQ>([&](auto Q) {
       constexpr int i = decltype(Q)::value;
-      lnact[i] = log(x[i]) + PF_LNGAM(i);
+      lnact[i] = pf_log(x[i]) + PF_LNGAM(i);
       // FP32 entries are accumulated pre-scaled by sig_i sig_j (any scale
       // works: it is divided out exactly below); FP64 ones are not scaled
       sig[i] = F32 ? __builtin_amdgcn_rsq(x[i]) : 1.0;
@@ -422,6 +460,11 @@ __device__ __forceinline__ void lane_step(const Args& A) {
     for (int q = 0; q < NT; ++q) S[q] = (JT)0;
     const bool wout = mode == 2;
     double nsI = 0.0, nss = 0.0;
+    double gi[NG1];
+    sfor<0, NG>([&](auto Q) {
+      constexpr int q = decltype(Q)::value;
+      gi[q] = PF_GINV(q);
+    });
     // complexes (RTotalAqueous, reaction.F90:4665-4759)
     sfor<0, NC>([&](auto K) {
       constexpr int k = decltype(K)::value;
@@ -432,16 +475,17 @@ __device__ __forceinline__ void lane_step(const Args& A) {
         constexpr int a = decltype(Q)::value;
         lnQK += Net::CPLX_NU[k][a] * lnact[Net::CPLX_ID[k][a]];
       });
-      double g;
+      double m;
+      double g = 1.0;
       if constexpr (NEWTON_ACT) {
         if constexpr (Net::CPLX_Z[k] > 1.0e-10 || Net::CPLX_Z[k] < -1.0e-10)
           g = dh_gamma(Net::CPLX_Z[k], Net::CPLX_A0[k], sI, I);
-        else
-          g = 1.0;
+        m = pf_exp(lnQK) / g;
+      } else if constexpr (Net::CPLX_GCLS[k] >= 0) {
+        m = pf_exp(lnQK) * gi[Net::CPLX_GCLS[k]];
       } else {
-        g = A.sec_act_coef[k * ld + c];
+        m = pf_exp(lnQK);
       }
-      const double m = NEWTON_ACT ? exp(lnQK) / g : exp(lnQK) * rcp_nr(g);
       if (wout) {
         A.sec_molal[k * ld + c] = m;
         if constexpr (NEWTON_ACT) A.sec_act_coef[k * ld + c] = g;
@@ -505,7 +549,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
       rt[mi] = 0.0;
       const double ar = PF_AREA(mi);
       if (ar != 0.0) {  // zero area gives Im = 0
-        const double QK = exp(lnQK);
+        const double QK = pf_exp(lnQK);
         const MRate r = mineral_rate<mi>(lnQK, QK, PF_VF(mi), ar, temp);
         if (r.active) {
           rt[mi] = r.Im;
@@ -560,7 +604,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
         A.pri_molal[i * ld + c] = x[i];
         const bool keep = ok && ((small >> i) & 1u);  // original value restored
         if (!keep) A.total[i * ld + c] = (!ok && restore) ? PF_TINIT(i) : totc[i];
-        if constexpr (NEWTON_ACT) A.pri_act_coef[i * ld + c] = exp(PF_LNGAM(i));
+        if constexpr (NEWTON_ACT) A.pri_act_coef[i * ld + c] = pf_exp(PF_LNGAM(i));
       });
       sfor<0, NIMM>([&](auto Q) {
         constexpr int i = NAQ + decltype(Q)::value;
@@ -712,7 +756,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
 #pragma unroll
           for (int i = 0; i < N; ++i) {
             const double u = copysign(1.0, delta[i]) * fmin(fabs(delta[i]), Net::MAX_DLNC);
-            latest[i] = x[i] * exp(-u);
+            latest[i] = x[i] * pf_exp(-u);
           }
         } else {
           double ratio = 1.0e308;
@@ -828,6 +872,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
 #undef PF_VF
 #undef PF_AREA
 #undef PF_RLAST
+#undef PF_GINV
 }
 
 }  // namespace pflane
