@@ -270,10 +270,6 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
       delete ctx;
       return PFCHEM_ERR_INVALID;
     }
-  if (opt->act_coef_update_algorithm == PFCHEM_ACT_COEF_ALGORITHM_NEWTON) {
-    delete ctx;
-    return PFCHEM_ERR_UNSUPPORTED;
-  }
   if (net->somdec) {
     const int rc = check_somdec(net, msg);
     if (rc) {
@@ -740,6 +736,13 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
   hipStream_t s = (hipStream_t)stream;
+  // RActivityCoefficients NEWTON inside every Newton iteration: only the
+  // one-cell-per-thread kernel carries that fixed point
+  if (variant != PFCHEM_KERNEL_PLAIN_FP64 &&
+      ctx->opt.act_coef_update_algorithm == PFCHEM_ACT_COEF_ALGORITHM_NEWTON &&
+      ctx->opt.act_coef_update_frequency == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER)
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED,
+                "the NEWTON activity algorithm per Newton iteration runs on the plain kernel");
   if (variant == PFCHEM_KERNEL_PLAIN_FP64) {
     // the one-cell-per-thread SomDec kernel faulted on MI355X without a
     // cause found in its code (its index checks pass on a host build); the

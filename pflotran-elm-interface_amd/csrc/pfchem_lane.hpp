@@ -96,11 +96,10 @@ __host__ __device__ constexpr int tri(int i, int j) {
   return i > j ? i * (i - 1) / 2 + j : j * (j - 1) / 2 + i;
 }
 
-// FP64 exp / log kept out of line: the network's complexes call them some
-// hundred times per Newton iteration, and one copy keeps the unrolled loop
-// inside the instruction cache
-__device__ __noinline__ double pf_exp(double v) { return exp(v); }
-__device__ __noinline__ double pf_log(double v) { return log(v); }
+// FP64 exp / log (inlined: out-of-line calls cost more than the
+// instruction-cache misses of the larger unrolled loop, measured on MI355X)
+__device__ __forceinline__ double pf_exp(double v) { return exp(v); }
+__device__ __forceinline__ double pf_log(double v) { return log(v); }
 
 // 1/g by the hardware reciprocal estimate refined with two Newton-Raphson
 // steps r <- r + r (1 - g r) (within about one ulp of the quotient; g > 0,

@@ -196,6 +196,111 @@ __device__ void plain_activity_lag(const DevNet& n, const DevState& s, int64_t c
   }
 }
 
+// RActivityCoefficients NEWTON (reaction.F90:4403-4551): fixed point on the
+// ionic strength I with the d I / d I correction of the secondary
+// molalities; the activities of the primaries stay those at entry, and every
+// pass recomputes the secondary molalities (written to the SoA state, as the
+// reference writes rt_auxvar%sec_molal).  More than 50 passes, or a negative
+// corrected I, is the reference's failure (:4420-4431, 4477-4486): free
+// molalities and coefficients become NaN and 1 is returned (the reference
+// keeps looping on NaN; this stops).
+template <int MAXN>
+__device__ int plain_activity_newton(const DevNet& n, const DevState& s, int64_t c,
+                                     PlainCell<MAXN>& w) {
+  const int naq = w.naq, ms = n.cplx_ms;
+  double lna[MAXN];
+  double sum_pri = 0.0, fpri = 0.0;
+  for (int i = 0; i < naq; ++i) {
+    lna[i] = log(w.x[i]) + log(w.gp[i]);
+    fpri += w.x[i] * n.Zp[i] * n.Zp[i];
+    if (i != n.h2o_aq_id) sum_pri += w.x[i];
+  }
+  double II = 0.0;
+  int err = 0;
+  for (int it = 1;; ++it) {
+    if (it > 50) err = 1;
+    double I = fpri;
+    for (int k = 0; k < n.nc; ++k) {
+      if (!plain_cplx_on(n, w, k)) continue;
+      I += s.sec_molal[k * s.ld + c] * n.cplx_Z[k] * n.cplx_Z[k];
+    }
+    I = 0.5 * I;
+    const double f = I;
+    if (fabs(I - II) < 1.0e-6 * I) break;
+    if (err) break;
+    if (n.nc > 0) {
+      double didi = 0.0;
+      const double sI = sqrt(I);
+      for (int k = 0; k < n.nc; ++k) {
+        if (!plain_cplx_on(n, w, k)) continue;
+        const double zk = n.cplx_Z[k];
+        if (fabs(zk) > 0.0) {
+          const double t = 1.0 + n.B * n.cplx_a0[k] * sI;
+          double sum = 0.5 * n.A * zk * zk / (sI * t * t) - n.Bdot;
+          const int* ids = n.cplx_ids + k * (ms + 1);
+          const double* st = n.cplx_st + k * ms;
+          for (int q = 0; q < ids[0]; ++q) {
+            const int j = ids[1 + q];
+            const double zj = n.Zp[j];
+            if (fabs(zj) > 0.0) {
+              const double tj = 1.0 + n.B * n.a0p[j] * sI;
+              sum += st[q] * (-0.5 * n.A * zj * zj / (sI * tj * tj) + n.Bdot);
+            }
+          }
+          didi += 0.5 * zk * zk * s.sec_molal[k * s.ld + c] * kLogToLn * sum;
+        }
+      }
+      const double den = 1.0 - didi;
+      II = fabs(den) > 0.0 ? (f - I * didi) / den : f;
+    } else {
+      II = f;
+    }
+    if (II < 0.0) err = 1;
+    I = II;
+    const double sI = sqrt(I);
+    for (int i = 0; i < naq; ++i)
+      w.gp[i] = fabs(n.Zp[i]) > 0.0 ? dh_gamma(n.Zp[i], n.a0p[i], sI, I, n) : 1.0;
+    double sum_sec = 0.0;
+    for (int k = 0; k < n.nc; ++k) {
+      if (!plain_cplx_on(n, w, k)) continue;
+      const double g = fabs(n.cplx_Z[k]) > 0.0 ? dh_gamma(n.cplx_Z[k], n.cplx_a0[k], sI, I, n) : 1.0;
+      s.sec_act_coef[k * s.ld + c] = g;
+      double logK = n.cplx_logK[k];
+      if (!n.isothermal && n.has_cplx_logKcoef) logK = interp_logk(n.cplx_logKcoef + 5 * k, w.temp);
+      double lnQK = -logK * kLogToLn;
+      if (n.has_cplx_h2o && n.cplx_h2o[k] != 0.0) lnQK += n.cplx_h2o[k] * w.ln_act_h2o;
+      const int* ids = n.cplx_ids + k * (ms + 1);
+      const double* st = n.cplx_st + k * ms;
+      for (int q = 0; q < ids[0]; ++q) lnQK += st[q] * lna[ids[1 + q]];
+      const double mk = exp(lnQK) / g;
+      s.sec_molal[k * s.ld + c] = mk;
+      sum_sec += mk;
+    }
+    if (n.act_h2o) {
+      const double a = 1.0 - 0.017 * (sum_pri + sum_sec);
+      w.ln_act_h2o = a > 0.0 ? log(a) : 0.0;
+    }
+  }
+  if (err) {
+    const double nan = __builtin_nan("");
+    for (int i = 0; i < naq; ++i) {
+      w.x[i] = nan;
+      w.gp[i] = nan;
+    }
+    for (int k = 0; k < n.nc; ++k)
+      if (plain_cplx_on(n, w, k)) s.sec_act_coef[k * s.ld + c] = nan;
+  }
+  return err;
+}
+
+// RActivityCoefficients with the network's algorithm; 1 on failure
+template <int MAXN>
+__device__ int plain_activity(const DevNet& n, const DevState& s, int64_t c, PlainCell<MAXN>& w) {
+  if (n.act_alg == PFCHEM_ACT_COEF_ALGORITHM_NEWTON) return plain_activity_newton(n, s, c, w);
+  plain_activity_lag(n, s, c, w);
+  return 0;
+}
+
 // Per-cell SomDec stoichiometry of one step: upstream N:C, mineralization C
 // and N coefficients, downstream N:C of every reaction (reset from the
 // descriptor's Setup values at the start of the step).
@@ -269,7 +374,8 @@ __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
   }
   for (;;) {
     ++*nit;
-    if (n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER) plain_activity_lag(n, s, c, w);
+    int act_err = 0;
+    if (n.act_freq == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER) act_err = plain_activity(n, s, c, w);
     plain_total(n, s, c, w, 1);
     if (*nit > n.max_it) {
       for (int i = 0; i < naq; ++i) w.total[i] = initial[i];
@@ -311,6 +417,9 @@ __device__ int plain_rreact(const DevNet& n, const DevState& s, int64_t c,
       }
       if constexpr (SD) plain_somdec(n, s, w, *sd, dt);
     }
+    // an activity-coefficient failure ends the RReact call (reaction.F90:
+    // 3922-3936)
+    if (act_err) return kActivity;
     double nr = 0.0, mx = 0.0;
     for (int i = 0; i < N; ++i) {
       nr += w.res[i] * w.res[i];
@@ -497,8 +606,11 @@ activity_kernel(const DevNet* np, DevState s, int64_t ncell) {
   w.N = w.naq + n.nimm;
   for (int i = 0; i < w.naq; ++i) w.x[i] = s.pri_molal[i * s.ld + c];
   w.ln_act_h2o = s.ln_act_h2o ? s.ln_act_h2o[c] : 0.0;
-  plain_activity_lag(n, s, c, w);
+  for (int i = 0; i < w.naq; ++i) w.gp[i] = s.pri_act_coef[i * s.ld + c];
+  const int err = plain_activity(n, s, c, w);
   for (int i = 0; i < w.naq; ++i) s.pri_act_coef[i * s.ld + c] = w.gp[i];
+  if (err)
+    for (int i = 0; i < w.naq; ++i) s.pri_molal[i * s.ld + c] = w.x[i];
   if (s.ln_act_h2o && n.act_h2o) s.ln_act_h2o[c] = w.ln_act_h2o;
 }
 
@@ -592,7 +704,7 @@ equilibrate_kernel(const DevNet* np, const DevConstraint* kp, DevState s, const 
       if (k.type[i] == PFCHEM_CONSTRAINT_FREE || k.type[i] == PFCHEM_CONSTRAINT_LOG)
         w.x[i] = free_conc[i];
     if (n.act_freq != PFCHEM_ACT_COEF_FREQUENCY_OFF && compute_act)  // :1632-1638
-      plain_activity_lag(n, s, c, w);
+      (void)plain_activity(n, s, c, w);
     plain_total(n, s, c, w, 1);  // :1639; dtotal (kg water / L) in J
     // rows of J: TOTAL keeps its dtotal row; the others are rebuilt.  The
     // charge-balance row needs every dtotal row, so it is built first.

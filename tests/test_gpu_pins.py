@@ -93,3 +93,49 @@ def test_update_activity_lag():
         e = float(G.rel_err(got[f], ref[f]).max())
         print("update_activity", f, e, flush=True)
         assert e <= 1e-12, (f, e)
+
+
+def test_update_activity_newton():
+    """pfchem_update_activity with the NEWTON algorithm (reaction.F90:4403-
+    4551) against the oracle: coefficients and the secondary molalities the
+    fixed point rewrites"""
+    import dataclasses
+    import torch
+    wl = workloads.random_network_workload(ncell=4096)
+    opt = dataclasses.replace(wl.opt, act_coef_update_algorithm=abi.ACT_COEF_ALGORITHM_NEWTON)
+    st = wl.state.copy()
+    st["pri_molal"][:, :] *= 1.3
+    ref = st.copy()
+    O.update_activity_batch(wl.net, opt, ref)
+    ctx = Context(wl.net, opt)
+    ds = DeviceState(st)
+    ctx.update_activity(ds)
+    torch.cuda.synchronize()
+    got = ds.to_host()
+    # cells where the fixed point fails (the reference's NaN outputs,
+    # reaction.F90:4420-4431) must be the same cells
+    fail_ref = np.isnan(ref["pri_act_coef"]).any(axis=0)
+    fail_gpu = np.isnan(got["pri_act_coef"]).any(axis=0)
+    print("update_activity NEWTON failed cells", int(fail_ref.sum()), int(fail_gpu.sum()), flush=True)
+    np.testing.assert_array_equal(fail_gpu, fail_ref)
+    ok = ~fail_ref
+    for f in ("pri_act_coef", "sec_act_coef", "sec_molal"):
+        e = float(G.rel_err(got[f][:, ok], ref[f][:, ok]).max())
+        print("update_activity NEWTON", f, e, flush=True)
+        assert e <= 1e-12, (f, e)
+
+
+def test_step_newton_activity_per_iteration():
+    """RStep with RActivityCoefficients NEWTON in every Newton iteration: the
+    plain kernel against the oracle; the batched kernels refuse the option"""
+    import dataclasses
+    wl = workloads.calcite_workload(1000)
+    opt = dataclasses.replace(wl.opt, act_coef_update_algorithm=abi.ACT_COEF_ALGORITHM_NEWTON,
+                              act_coef_update_frequency=abi.ACT_COEF_FREQUENCY_NEWTON_ITER)
+    wl = dataclasses.replace(wl, opt=opt)
+    _parity(wl, abi.KERNEL_PLAIN_FP64, None, "newton-activity/plain")
+    from pfchem.lib import PfchemError
+    ctx = Context(wl.net, wl.opt)
+    with pytest.raises(PfchemError) as e:
+        ctx.step(DeviceState(wl.state), wl.dt, abi.KERNEL_BATCHED_FP64)
+    assert e.value.code == abi.PFCHEM_ERR_UNSUPPORTED
