@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the per-config lines (C1, C3, C4, C5)")
     ap.add_argument("--only", choices=["plain", "fp64", "mixed", "team-fp64", "team-mixed"],
                     help="time only this kernel of the step (profiling runs) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -352,8 +354,171 @@ def main():
         "newton_iterations_mean": float(iters_gpu.mean()),
     }
     out.update(parity)
+    if not args.no_configs:
+        out["configs"] = run_configs(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _timed_steps(ctx, dev0, dt, variant, nsteps, nwarm):
+    """(last state, ms per step) of nsteps steps, each on its own copy of
+    the HBM-resident state, after nwarm untimed ones"""
+    import torch
+    from pfchem.lib import DeviceState
+    out = None
+    for i in range(nwarm + nsteps):
+        d = DeviceState.__new__(DeviceState)
+        d.net, d.ncell, d.ld, d.fields = dev0.net, dev0.ncell, dev0.ld, dev0.fields
+        d.t = {name: t.clone() for name, t in dev0.t.items()}
+        torch.cuda.synchronize()
+        if i == nwarm:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            total = 0.0
+        if i >= nwarm:
+            e0.record()
+        ctx.step(d, dt, variant)
+        if i >= nwarm:
+            e1.record()
+            torch.cuda.synchronize()
+            total += e0.elapsed_time(e1)
+        out = d
+    torch.cuda.synchronize()
+    return out, total / nsteps
+
+
+def _conc_err(ref, got, cells, fields=("pri_molal", "sec_molal", "total", "immobile")):
+    import numpy as np
+    worst = {}
+    for f in fields:
+        a = ref[f][:, cells]
+        b = got[f][:, cells]
+        if a.size == 0:
+            continue
+        floor = 1e-12 * np.abs(a).max(axis=1, keepdims=True) if f == "total" else 0.0
+        d = np.abs(b - a)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(d == 0, 0.0, d / np.maximum(np.abs(a), floor))
+        worst[f] = float(r.max())
+    return worst
+
+
+def run_configs(args):
+    """The configurations besides the headline: cells/s of the plain FP64,
+    batched FP64 and batched mixed steps, their concentration error against
+    the FP64 oracle on a sample of cells, and the oracle's cells/s on that
+    sample (host threads stated); for the adjoint configuration the
+    pfchem_adjoint time next to the step it differentiates."""
+    import numpy as np
+    import torch
+    import oracle_lib as O
+    from pfchem import abi, workloads
+    from pfchem.lib import Context, DeviceState
+    nthr = (args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+            or min(16, os.cpu_count() or 1))
+    res = {}
+
+    def leg(key, wl, sample, variants, plain_steps=1, steps=3, extra=None, oracle=None):
+        t0 = time.time()
+        n = wl.state.ncell
+        ctx = Context(wl.net, wl.opt)
+        dev0 = DeviceState(wl.state)
+        line = {"workload": wl.notes[:160], "cells": n, "dt_s": wl.dt}
+        rng = np.random.default_rng(7)
+        cells = np.sort(rng.choice(n, size=min(sample, n), replace=False))
+        sub = wl.state.subset(cells)
+        t1 = time.perf_counter()
+        if oracle is None:
+            ref = sub.copy()
+            nfail = O.step_batch(wl.net, wl.opt, ref, wl.dt, nthr)
+        else:
+            ref = oracle(wl, sub, nthr)
+            nfail = int((ref["status"][0] != 0).sum())
+        t_cpu = time.perf_counter() - t1
+        line["cpu_oracle"] = {"cells_per_s": cells.size / t_cpu, "threads": nthr,
+                              "sample_cells": int(cells.size), "failed_cells": int(nfail)}
+        okref = ref["status"][0] == 0
+        for v in variants:
+            name = abi.KERNEL_NAMES[v]
+            try:
+                st, ms = _timed_steps(ctx, dev0, wl.dt, v,
+                                      plain_steps if v == abi.KERNEL_PLAIN_FP64 else steps, 1)
+            except Exception as e:  # a variant the network does not support
+                line[name] = {"unsupported": str(e)[:200]}
+                continue
+            got = st.to_host()
+            g = got.subset(cells)
+            ok = okref & (g["status"][0] == 0)
+            line[name] = {"ms_per_step": ms, "cells_per_s": n / (ms * 1e-3),
+                          "failed_cells": int((got["status"][0, :n] != 0).sum()),
+                          "launch": ctx.get_launch(v),
+                          "max_rel_conc_err": _conc_err(ref, g, np.where(ok)[0])}
+            del st, got
+        p = line.get(abi.KERNEL_NAMES[abi.KERNEL_PLAIN_FP64], {})
+        m = line.get(abi.KERNEL_NAMES[abi.KERNEL_BATCHED_MIXED], {})
+        if "cells_per_s" in p and "cells_per_s" in m:
+            line["speedup_mixed_vs_plain"] = m["cells_per_s"] / p["cells_per_s"]
+        if extra:
+            extra(ctx, dev0, wl, line, cells, sub)
+        line["wall_s"] = time.time() - t0
+        log("[bench] %s: %s" % (key, json.dumps(line)))
+        res[key] = line
+        del ctx, dev0
+        torch.cuda.empty_cache()
+
+    allv = [abi.KERNEL_PLAIN_FP64, abi.KERNEL_BATCHED_FP64, abi.KERNEL_BATCHED_MIXED]
+    # C1: the calcite system, 10x10x10 cells
+    leg("C1_calcite_1000", workloads.calcite_workload(1000), 1000, allv, plain_steps=3, steps=5)
+
+    # C4: adjoint of the step on the 15-primary network (65 parameters:
+    # 5 log10 k + 60 log10 K), 262,144 cells; objective: sum of end-of-step
+    # mineral volume fractions + the secondary molality of complex 0
+    def adjoint_leg(ctx, dev0, wl, line, cells, sub):
+        wv = np.ones(wl.net.nkinmnrl)
+        ws = np.zeros(wl.net.neqcplx)
+        ws[0] = 1.0
+        ds = DeviceState.__new__(DeviceState)
+        ds.net, ds.ncell, ds.ld, ds.fields = dev0.net, dev0.ncell, dev0.ld, dev0.fields
+        ds.t = dict(dev0.t)
+        ctx.adjoint(ds, wl.dt, wv, ws)
+        torch.cuda.synchronize()
+        reps = 3
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            grad, f, st, _ = ctx.adjoint(ds, wl.dt, wv, ws)
+        ms = (time.perf_counter() - t1) / reps * 1e3
+        sg = ctx.adjoint(ds.__class__(sub), wl.dt, wv, ws)[0]
+        t1 = time.perf_counter()
+        rg, _, rst = O.adjoint_batch(wl.net, wl.opt, sub, wl.dt, wv, ws, True, nthr)
+        t_cpu = time.perf_counter() - t1
+        scale = np.abs(rg).max()
+        line["adjoint"] = {"ms": ms, "cells_per_s": wl.state.ncell / (ms * 1e-3),
+                           "parameters": int(wv.size + ws.size),
+                           "sample_max_rel_grad_err": float(np.abs(sg - rg).max() / scale),
+                           "cpu_oracle_cells_per_s": sub.ncell / t_cpu,
+                           "status_ok_cells": int((st == 0).sum().item())}
+        mixed = line.get(abi.KERNEL_NAMES[abi.KERNEL_BATCHED_MIXED], {})
+        if "ms_per_step" in mixed:
+            line["mixed_step_plus_adjoint_cells_per_s"] = (
+                wl.state.ncell / ((mixed["ms_per_step"] + ms) * 1e-3))
+
+    leg("C4_adjoint_random15_262144", workloads.random_network_workload(ncell=262144), 4096,
+        allv, extra=adjoint_leg)
+    # C5: 64 primaries, 300 complexes, 10 minerals; per-cell active counts
+    # per-cell active counts: the oracle runs each cell on its restricted
+    # sub-network (cells grouped by active count)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import variable_util as V
+
+    def variable_oracle(wl, sub, nthr):
+        import dataclasses
+        return V.run_oracle_variable(dataclasses.replace(wl, state=sub), nthreads=nthr)[0]
+
+    leg("C5_variable_262144", workloads.variable_network_workload(ncell=262144), 2048, allv,
+        plain_steps=1, steps=2, oracle=variable_oracle)
+    # C3: SomDec decomposition, 500,000 columns x 15 layers
+    leg("C3_somdec_7500000", workloads.somdec_workload(), 32768, allv, steps=2)
+    return res
 
 
 if __name__ == "__main__":

@@ -32,6 +32,7 @@ from dataclasses import dataclass
 from typing import Optional
 
 import dataclasses
+import os
 
 import numpy as np
 
@@ -96,9 +97,15 @@ def _lag_gamma(net: ReactionNetwork, m: np.ndarray, sec: np.ndarray):
     sI = np.sqrt(I)
 
     def g(Z, a0):
-        lg = -Z * Z * sI * net.debyeA / (1.0 + a0 * net.debyeB * sI) + net.debyeBdot * I
-        out = np.exp(lg * LOG_TO_LN)
-        return np.where(np.abs(Z) > 1e-10, out, 1.0)
+        # one evaluation per distinct (Z^2, a0): the same values, computed once
+        Z = Z[:, 0]
+        keys = np.stack([Z * Z, np.broadcast_to(a0[:, 0], Z.shape)], axis=1)
+        uniq, inv = np.unique(keys, axis=0, return_inverse=True)
+        uz = np.sqrt(uniq[:, 0])[:, None]
+        ua = uniq[:, 1][:, None]
+        lg = -uz * uz * sI * net.debyeA / (1.0 + ua * net.debyeB * sI) + net.debyeBdot * I
+        out = np.exp(lg * LOG_TO_LN)[inv.reshape(-1)]
+        return np.where(np.abs(Z)[:, None] > 1e-10, out, 1.0)
 
     return g(Zp, net.primary_spec_a0[:, None]), g(Zc, net.eqcplx_a0[:, None]), I
 
@@ -118,6 +125,21 @@ def _speciate(net: ReactionNetwork, m: np.ndarray, gp: np.ndarray, gc: np.ndarra
 
 
 def _equilibrate_gamma(net, m, passes=3):
+    """column-parallel over cell chunks (element-wise numpy work releases
+    the GIL; every cell's result is independent of the chunking)"""
+    n = m.shape[1]
+    chunk = 32768
+    if n <= chunk:
+        return _equilibrate_gamma_cols(net, m, passes)
+    from concurrent.futures import ThreadPoolExecutor
+    parts = [(a, min(n, a + chunk)) for a in range(0, n, chunk)]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(lambda ab: _equilibrate_gamma_cols(net, m[:, ab[0]:ab[1]], passes),
+                          parts))
+    return tuple(np.concatenate([r[q] for r in res], axis=1) for q in range(4))
+
+
+def _equilibrate_gamma_cols(net, m, passes=3):
     sec = np.zeros((net.neqcplx, m.shape[1]))
     gp = np.ones_like(m)
     gc = np.ones_like(sec)
@@ -133,12 +155,14 @@ def _draw_positive_totals(net, draw, ncell, rounds=200):
     positive.  RStep replaces totals <= 1e-40 by 1e-40 (reaction.F90:3617-3631),
     which would turn a negative total into a different problem."""
     m0 = draw(ncell)
+    check = np.arange(ncell)  # cells drawn since their totals were checked
     for _ in range(rounds):
-        _, T0, _, _ = _equilibrate_gamma(net, m0)
-        bad = np.where((T0 <= 0.0).any(axis=0))[0]
+        _, T0, _, _ = _equilibrate_gamma(net, m0[:, check])
+        bad = check[np.where((T0 <= 0.0).any(axis=0))[0]]
         if bad.size == 0:
             return m0
         m0[:, bad] = draw(bad.size)
+        check = bad
     raise RuntimeError("could not draw cells with positive totals")
 
 
@@ -352,7 +376,11 @@ def variable_network_workload(ncell: int = 262_144, seed: int = 20261206, dt: fl
                                  variable_ncomp=True)
     rng = _rng(seed + 2)
     wl.state["ncomp_active"][0, :ncell] = rng.integers(nmin, 65, ncell)
-    opt = dataclasses.replace(wl.opt, variable_ncomp=True)
+    # relative-residual tolerance 1e-13: sums of up to 64 terms leave some
+    # cells' residuals above 1e-14 of their initial value at the rounding
+    # level, so at 1e-14 their Newton iteration would stop only by cutting
+    # the step
+    opt = dataclasses.replace(wl.opt, variable_ncomp=True, max_rel_residual_tolerance=1.0e-13)
     notes = wl.notes + ("; 64 primaries, 300 complexes, 10 minerals; per-cell active "
                         "primaries U{%d..64} (prefix 0..nact-1)" % nmin)
     return Workload("variable_n64_c300_m10", wl.net, opt, wl.state, dt, seed, notes)
