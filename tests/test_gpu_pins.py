@@ -32,7 +32,7 @@ KERNELS = [(abi.KERNEL_PLAIN_FP64, None), (abi.KERNEL_BATCHED_FP64, None),
 IDS = ["plain", "lane-fp64", "lane-mixed", "team16-fp64", "team16-mixed"]
 
 
-def _parity(wl, variant, lanes, label):
+def _parity(wl, variant, lanes, label, residual=True):
     ref, rec, nfail = G.run_oracle(wl)
     assert nfail == 0
     print("case", label, flush=True)
@@ -45,8 +45,9 @@ def _parity(wl, variant, lanes, label):
     print(label, errs, flush=True)
     for f, e in errs.items():
         assert e <= CONC_TOL, (label, f, e)
-    rr, bar = G.gate_rel_residual(wl, got, rec, range(n), ref)
-    assert rr.size > 0.8 * n and np.all(rr <= bar), label
+    if residual:
+        rr, bar = G.gate_rel_residual(wl, got, rec, range(n), ref)
+        assert rr.size > 0.8 * n and np.all(rr <= bar), label
 
 
 @pytest.mark.parametrize("variant,lanes", KERNELS, ids=IDS)
@@ -133,7 +134,10 @@ def test_step_newton_activity_per_iteration():
     opt = dataclasses.replace(wl.opt, act_coef_update_algorithm=abi.ACT_COEF_ALGORITHM_NEWTON,
                               act_coef_update_frequency=abi.ACT_COEF_FREQUENCY_NEWTON_ITER)
     wl = dataclasses.replace(wl, opt=opt)
-    _parity(wl, abi.KERNEL_PLAIN_FP64, None, "newton-activity/plain")
+    # with the coefficients recomputed inside every Newton iteration the
+    # recorded sub-step system is not a fixed function of the free
+    # molalities, so only the concentrations are compared
+    _parity(wl, abi.KERNEL_PLAIN_FP64, None, "newton-activity/plain", residual=False)
     from pfchem.lib import PfchemError
     ctx = Context(wl.net, wl.opt)
     with pytest.raises(PfchemError) as e:
