@@ -481,6 +481,21 @@ typedef struct pfchem_constraint {
 int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                        const pfchem_constraint* con, const double* conc, void* stream);
 
+/* pfchem_equilibrate without any allocation in the call: no reference
+   counterpart; for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   The constraint descriptor, its reaction table and the failure count are
+   staged in a fixed 32 KiB device array of the library (one per GPU, held
+   until the process ends) instead of context memory grown on demand, and
+   the host image is built on the stack.  Same arguments, results and
+   return codes as pfchem_equilibrate; a constraint whose staging exceeds
+   32 KiB (about 190 reaction rows of 8 species) gives
+   PFCHEM_ERR_UNSUPPORTED.  Every call on one GPU shares that array:
+   overlapping calls on one GPU (other host threads or streams) race; each
+   call clears its failure count and blocks until its kernel finished. */
+int pfchem_equilibrate_static(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                              const pfchem_constraint* con, const double* conc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,33 @@ struct Packer {
 };
 
 const size_t kNone = ~size_t(0);
+
+// Fixed device array for pfchem_equilibrate_static: the constraint
+// descriptor, its reaction table and the failure count.  Each GPU holds its
+// own copy for the life of the process; host code reaches it only through
+// the address hipGetSymbolAddress returns on that device.
+constexpr size_t kStaticScratchBytes = 32768;
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// resolve the array's address on `device` (current); once per device
+hipError_t resolve_static_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+  return e;
+}
+
+void* static_scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 // Launches take their state descriptor from a ring of device slots, filled
 // from a pinned host mirror by a copy on the launch's stream.  An event is
 // recorded after every launch that reads a slot; before the slot (and its
@@ -294,6 +322,10 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     return PFCHEM_ERR_HIP;
   }
   ctx->device = device;
+  if (resolve_static_scratch(device) != hipSuccess) {
+    delete ctx;
+    return PFCHEM_ERR_HIP;
+  }
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       cus > 0)
@@ -1124,8 +1156,10 @@ int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
 extern "C" {
 
 // ReactionEquilibrateConstraint over ncell cells (reaction.F90:1328-2117)
-int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
-                       const pfchem_constraint* con, const double* conc, void* stream) {
+// Argument checks of pfchem_equilibrate / pfchem_equilibrate_static; on
+// success *d holds the state descriptor
+static int equilibrate_check(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                             const pfchem_constraint* con, const double* conc, DevState* dout) {
   if (!ctx) return PFCHEM_ERR_INVALID;
   if (ncell < 0) return fail(ctx, PFCHEM_ERR_INVALID, "ncell < 0");
   if (!con || !con->constraint_type || !con->constraint_rxn)
@@ -1179,20 +1213,90 @@ int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state
         return fail(ctx, PFCHEM_ERR_INVALID, "unknown constraint type");
     }
   }
-  DevState d;
-  int rc = to_dev_state(ctx, state, ncell, &d);
-  if (rc) return rc;
-  if (ncell == 0) return PFCHEM_OK;
-  // one buffer: DevConstraint | ids | stoich | h2o | logK | coef | nfail
+  return to_dev_state(ctx, state, ncell, dout);
+}
+
+// Byte layout of the constraint staging buffer: DevConstraint | ids | stoich
+// | h2o | logK | coef | failure count (256-byte aligned parts)
+struct ConLayout {
+  size_t o_ids, o_st, o_h2o, o_lk, o_cf, o_nf, bytes;
+};
+static ConLayout con_layout(int nr, int ms) {
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int rows = nr > 0 ? nr : 1, rms = nr > 0 ? ms : 1;
-  const size_t o_ids = up(sizeof(DevConstraint));
-  const size_t o_st = o_ids + up(sizeof(int) * rows * (rms + 1));
-  const size_t o_h2o = o_st + up(sizeof(double) * rows * rms);
-  const size_t o_lk = o_h2o + up(sizeof(double) * rows);
-  const size_t o_cf = o_lk + up(sizeof(double) * rows);
-  const size_t o_nf = o_cf + up(sizeof(double) * rows * 5);
-  const size_t bytes = o_nf + 256;
+  ConLayout L;
+  L.o_ids = up(sizeof(DevConstraint));
+  L.o_st = L.o_ids + up(sizeof(int) * rows * (rms + 1));
+  L.o_h2o = L.o_st + up(sizeof(double) * rows * rms);
+  L.o_lk = L.o_h2o + up(sizeof(double) * rows);
+  L.o_cf = L.o_lk + up(sizeof(double) * rows);
+  L.o_nf = L.o_cf + up(sizeof(double) * rows * 5);
+  L.bytes = L.o_nf + 256;
+  return L;
+}
+
+// Fill the host image h (L.bytes, zeroed) of the staging buffer that will
+// live at device address dbase; the failure count starts at zero
+static void con_image(const DevNet& n, const pfchem_constraint* con, const ConLayout& L,
+                      char* dbase, char* h) {
+  const int nr = con->nrxn, ms = con->rxn_maxsp;
+  DevConstraint k{};
+  for (int i = 0; i < n.naq; ++i) {
+    k.type[i] = con->constraint_type[i];
+    k.rxn[i] = con->constraint_rxn[i];
+    k.guess[i] = con->free_ion_guess ? con->free_ion_guess[i] : 1.0e-9;
+  }
+  k.nrxn = nr;
+  k.ms = nr > 0 ? ms : 1;
+  k.h_ion_id = con->h_ion_id;
+  k.with_molality = con->initialize_with_molality != 0;
+  k.use_prev = con->use_prev_soln_as_guess != 0;
+  k.max_it = con->max_iterations;
+  k.has_guess = con->free_ion_guess != nullptr;
+  k.has_h2o = nr > 0 && con->rxnh2ostoich != nullptr;
+  k.has_coef = nr > 0 && con->rxn_logKcoef != nullptr;
+  k.max_dlnC = con->max_dlnC;
+  k.ids = (const int*)(dbase + L.o_ids);
+  k.st = (const double*)(dbase + L.o_st);
+  k.h2o = (const double*)(dbase + L.o_h2o);
+  k.logK = (const double*)(dbase + L.o_lk);
+  k.coef = (const double*)(dbase + L.o_cf);
+  k.nfail = (int*)(dbase + L.o_nf);
+  std::memcpy(h, &k, sizeof(k));
+  if (nr > 0) {
+    std::memcpy(h + L.o_ids, con->rxnspecid, sizeof(int) * nr * (ms + 1));
+    std::memcpy(h + L.o_st, con->rxnstoich, sizeof(double) * nr * ms);
+    if (k.has_h2o) std::memcpy(h + L.o_h2o, con->rxnh2ostoich, sizeof(double) * nr);
+    std::memcpy(h + L.o_lk, con->rxn_logK, sizeof(double) * nr);
+    if (k.has_coef) std::memcpy(h + L.o_cf, con->rxn_logKcoef, sizeof(double) * nr * 5);
+  }
+}
+
+// Upload the image, launch, read the failure count back and synchronize
+static int con_run(pfchem_ctx* ctx, int64_t ncell, const DevState& d, const double* conc,
+                   const ConLayout& L, char* dbase, const char* h, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(dbase, h, L.bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "constraint copy");
+  e = launch_equilibrate(ctx->dnet, ctx->dnet_dev, (const DevConstraint*)dbase, d, conc, ncell, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "equilibrate launch");
+  int nfail = 0;
+  e = hipMemcpyAsync(&nfail, dbase + L.o_nf, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "failure count copy");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(ctx, e, "equilibrate");
+  if (nfail > 0)
+    return fail(ctx, PFCHEM_ERR_CELLS, std::to_string(nfail) + " cell(s) failed to equilibrate");
+  return PFCHEM_OK;
+}
+
+int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                       const pfchem_constraint* con, const double* conc, void* stream) {
+  DevState d;
+  int rc = equilibrate_check(ctx, ncell, state, con, conc, &d);
+  if (rc) return rc;
+  if (ncell == 0) return PFCHEM_OK;
+  const ConLayout L = con_layout(con->nrxn, con->rxn_maxsp);
+  const size_t bytes = L.bytes;
   hipError_t e = hipSetDevice(ctx->device);
   if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
   hipStream_t s = (hipStream_t)stream;
@@ -1211,49 +1315,33 @@ int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state
     ctx->con_bytes = bytes;
   }
   std::vector<char> h(bytes, 0);
-  char* dbase = (char*)ctx->con_buf;
-  DevConstraint k{};
-  for (int i = 0; i < n.naq; ++i) {
-    k.type[i] = con->constraint_type[i];
-    k.rxn[i] = con->constraint_rxn[i];
-    k.guess[i] = con->free_ion_guess ? con->free_ion_guess[i] : 1.0e-9;
-  }
-  k.nrxn = nr;
-  k.ms = rms;
-  k.h_ion_id = con->h_ion_id;
-  k.with_molality = con->initialize_with_molality != 0;
-  k.use_prev = con->use_prev_soln_as_guess != 0;
-  k.max_it = con->max_iterations;
-  k.has_guess = con->free_ion_guess != nullptr;
-  k.has_h2o = nr > 0 && con->rxnh2ostoich != nullptr;
-  k.has_coef = nr > 0 && con->rxn_logKcoef != nullptr;
-  k.max_dlnC = con->max_dlnC;
-  k.ids = (const int*)(dbase + o_ids);
-  k.st = (const double*)(dbase + o_st);
-  k.h2o = (const double*)(dbase + o_h2o);
-  k.logK = (const double*)(dbase + o_lk);
-  k.coef = (const double*)(dbase + o_cf);
-  k.nfail = (int*)(dbase + o_nf);
-  std::memcpy(h.data(), &k, sizeof(k));
-  if (nr > 0) {
-    std::memcpy(h.data() + o_ids, con->rxnspecid, sizeof(int) * nr * (ms + 1));
-    std::memcpy(h.data() + o_st, con->rxnstoich, sizeof(double) * nr * ms);
-    if (k.has_h2o) std::memcpy(h.data() + o_h2o, con->rxnh2ostoich, sizeof(double) * nr);
-    std::memcpy(h.data() + o_lk, con->rxn_logK, sizeof(double) * nr);
-    if (k.has_coef) std::memcpy(h.data() + o_cf, con->rxn_logKcoef, sizeof(double) * nr * 5);
-  }
-  e = hipMemcpyAsync(ctx->con_buf, h.data(), bytes, hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) return hip_fail(ctx, e, "constraint copy");
-  e = launch_equilibrate(n, ctx->dnet_dev, (const DevConstraint*)ctx->con_buf, d, conc, ncell, s);
-  if (e != hipSuccess) return hip_fail(ctx, e, "equilibrate launch");
-  int nfail = 0;
-  e = hipMemcpyAsync(&nfail, dbase + o_nf, sizeof(int), hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return hip_fail(ctx, e, "failure count copy");
-  e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hip_fail(ctx, e, "equilibrate");
-  if (nfail > 0)
-    return fail(ctx, PFCHEM_ERR_CELLS, std::to_string(nfail) + " cell(s) failed to equilibrate");
-  return PFCHEM_OK;
+  con_image(ctx->dnet, con, L, (char*)ctx->con_buf, h.data());
+  return con_run(ctx, ncell, d, conc, L, (char*)ctx->con_buf, h.data(), s);
+}
+
+// The same speciation with its staging buffer in the library's fixed device
+// array (no allocation in the call; see pfchem.h)
+int pfchem_equilibrate_static(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                              const pfchem_constraint* con, const double* conc, void* stream) {
+  DevState d;
+  int rc = equilibrate_check(ctx, ncell, state, con, conc, &d);
+  if (rc) return rc;
+  const ConLayout L = con_layout(con->nrxn, con->rxn_maxsp);
+  if (L.bytes > kStaticScratchBytes)
+    return fail(ctx, PFCHEM_ERR_UNSUPPORTED,
+                "constraint reaction table too large for the fixed staging array (" +
+                    std::to_string(L.bytes) + " > " + std::to_string(kStaticScratchBytes) +
+                    " bytes; use pfchem_equilibrate)");
+  if (ncell == 0) return PFCHEM_OK;
+  char* dbase = (char*)static_scratch_ptr(ctx->device);
+  if (!dbase) return fail(ctx, PFCHEM_ERR_HIP, "fixed staging array not resolved on this device");
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+  // host image on the stack: no heap allocation either
+  alignas(256) char h[kStaticScratchBytes];
+  std::memset(h, 0, L.bytes);
+  con_image(ctx->dnet, con, L, dbase, h);
+  return con_run(ctx, ncell, d, conc, L, dbase, h, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -50,6 +50,7 @@ def load(path: str = LIB_PATH):
     L.pfchem_residual.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp]
     L.pfchem_residual_jacobian.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, vp]
     L.pfchem_equilibrate.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+    L.pfchem_equilibrate_static.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
     L.pfchem_set_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_get_launch.argtypes = [vp, C.c_int, vp]
     L.pfchem_tangent.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, vp, C.c_int, vp, vp, vp,
