@@ -75,6 +75,16 @@ def main(ref):
     golds["clm_cn3"] = {"source": deck + "-obs-0.tec.gold",
                         "columns": header, "rows": rows,
                         "final": parse_gold(os.path.join(ref, deck + ".regression.gold"))}
+    # one-reaction SOM decks with N2O emission: end-of-run regression values;
+    # clm_som1 also the step times of its observation series (every step)
+    deck = "regression_tests/ngee/CLMCNplus/clm_som1"
+    header, rows = parse_obs(os.path.join(ref, deck + "-obs-0.tec.gold"), 10 ** 6)
+    golds["clm_som1"] = {"source": deck + ".regression.gold",
+                         "obs_times_d": [r[0] for r in rows],
+                         "final": parse_gold(os.path.join(ref, deck + ".regression.gold"))}
+    deck = "regression_tests/ngee/CLMCNplus/clm_nmin"
+    golds["clm_nmin"] = {"source": deck + ".regression.gold",
+                         "final": parse_gold(os.path.join(ref, deck + ".regression.gold"))}
     for case, (gold, db) in CASES.items():
         golds[case] = {"source": gold, "database": os.path.basename(db),
                        "values": parse_gold(os.path.join(ref, gold))}

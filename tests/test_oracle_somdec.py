@@ -1,5 +1,6 @@
 """The oracle's SomDec restatement (oracle/pfchem_oracle_somdec.c) against the
-reference's clm_cn3 regression deck and against conservation of C and N.
+reference's clm_cn3, clm_som1 and clm_nmin regression decks, the closed-form
+Q10 response, and conservation of C and N.
 
 clm_cn3 (regression_tests/ngee/CLMCNplus) is a one-cell GIRT run of the
 CLM-CN litter/SOM cascade with Monod NH4+/NO3- limitation; its observation
@@ -94,3 +95,145 @@ def test_somdec_moisture_zero_stops_decomposition():
     assert O.step_batch(wl.net, wl.opt, st, wl.dt, 0) == 0
     np.testing.assert_allclose(st.arrays["immobile"][:, :64], wl.state.arrays["immobile"][:, :64],
                                rtol=1e-12)
+
+
+# One-reaction SOM decks with N2O emission (regression_tests/ngee/CLMCNplus).
+# Both are one-cell GIRT runs without flow, so every time step solves the
+# RReact system of the cell: the replay takes the decks' step sizes and
+# compares the end-of-run regression values (14 digits).  The ratio of N2O to
+# the mineralized N is f_T(25 C) f_w(s=1) f_pH x N2O_FRAC_MINERALIZATION
+# (SomDecNemission, reaction_sandbox_somdec.F90:3477-3640), so these pin the
+# emission factors; NH4+ pins the mineralization stoichiometry (SomDecReact1,
+# :1914-2160) and the tracking species Nmin / HRimm / NGASmin.
+from pfchem import somdec as SD
+from pfchem.network import ReactionNetwork
+
+DAY = 86400.0
+
+
+def _som_cell(prim, imm, pools, reactions, total0, immobile0, **build):
+    sd = SD.build_somdec(pools, reactions, prim, imm, **build)
+    net = ReactionNetwork(primary_species_names=list(prim), primary_spec_Z=np.zeros(len(prim)),
+                          primary_spec_a0=np.zeros(len(prim)), immobile_names=list(imm),
+                          somdec=sd)
+    opt = SolverOptions(act_coef_update_algorithm=abi.ACT_COEF_ALGORITHM_LAG,
+                        max_residual_tolerance=1e-30, max_rel_residual_tolerance=1e-14,
+                        max_relative_change_tolerance=1e-14, maximum_reaction_iterations=100)
+    cc = O.CellCtx(net, opt)
+    cc.c.den_kg = 1000.0          # REFERENCE_LIQUID_DENSITY 1.d3
+    cc.c.porosity = 0.25
+    cc.c.sat = 1.0
+    cc.c.volume = 1.0
+    cc.c.temp = 25.0
+    cc.set("total", total0)
+    cc.set("pri_molal", total0)   # no complexes, unit activity
+    cc.set("pri_act_coef", np.ones(len(prim)))
+    cc.set("immobile", immobile0)
+    return cc
+
+
+def _clm_som1(n2o_frac=0.02, x0eps=1.0e-20):
+    """clm_som1.in: SOM1 -> 0.72 SOM2 (both C:N 12), 0.7 /d, pH 6"""
+    prim = ["CO2(aq)", "N2O(aq)", "NH4+", "NO3-", "H+"]
+    imm = ["SOM1", "SOM2"]
+    cc = _som_cell(prim, imm, [SD.Pool("SOM1", 12.0), SD.Pool("SOM2", 12.0)],
+                   [SD.Reaction("SOM1", [("SOM2", 0.72)], rate_constant=0.7 / DAY)],
+                   np.array([1e-20, 1e-40, 1e-8, 1e-20, 1e-6]), [1.0, 1e-20],
+                   n2o_frac_mineralization=n2o_frac, x0eps=x0eps)
+    times = np.array(GU.golds()["clm_som1"]["obs_times_d"]) * DAY
+    # the observation series records every step, times to 7 digits; the
+    # deck's steps (1 h, 2 h, then the 0.1 d maximum, the last one cut to
+    # land on 200 d) are multiples of 360 s
+    dts = np.round(np.diff(times) / 360.0) * 360.0
+    for dt in dts:
+        err, nit, why = cc.rreact(float(dt))
+        assert err == 0
+    tot = cc.get("total", 5)
+    im = cc.get("immobile", 2)
+    return dts, {"Total CO2(aq)": tot[0], "Total N2O(aq)": tot[1], "Total NH4+": tot[2],
+                 "SOM1": im[0], "SOM2": im[1]}
+
+
+def _gold_rel(case, got):
+    fin = GU.golds()[case]["final"]
+    return {k: abs(v - fin["CONCENTRATION: " + k]) / abs(fin["CONCENTRATION: " + k])
+            for k, v in got.items()}
+
+
+def test_clm_som1_matches_the_gold():
+    dts, got = _clm_som1()
+    fin = GU.golds()["clm_som1"]["final"]
+    assert len(dts) == fin["SOLUTION: Transport: Time Steps"]
+    assert dts[0] == 3600.0 and dts[1] == 7200.0 and dts[2] == 8640.0
+    rel = _gold_rel("clm_som1", got)
+    print("clm_som1", rel)
+    # the deck's Newton stops at PETSc's default relative tolerance (1e-8)
+    for k in ("Total CO2(aq)", "Total N2O(aq)", "Total NH4+", "SOM2"):
+        assert rel[k] < 5e-8, (k, rel[k])
+    # SOM1 ends on the plateau that the x0eps smoothing of the upstream pool
+    # leaves (1.26e-20 mol/m^3), where each step moves it by ~1e-6
+    assert rel["SOM1"] < 1e-4, rel["SOM1"]
+
+
+def test_clm_som1_gold_catches_wrong_emission_and_smoothing():
+    """a 0.5% slip in the N2O fraction, or the smoothing floor a decade off,
+    misses the gold"""
+    _, got = _clm_som1(n2o_frac=0.0201)
+    assert _gold_rel("clm_som1", got)["Total N2O(aq)"] > 1e-3
+    _, got = _clm_som1(x0eps=1.0e-21)
+    assert _gold_rel("clm_som1", got)["SOM1"] > 0.5
+
+
+def _nmin_steps():
+    """clm_nmin.in: 1 h initial step doubling to the 1 d maximum, the last
+    step cut to land on 365 d"""
+    h = 3600.0
+    dts = [h * 2 ** k for k in range(5)] + [DAY] * 363
+    dts.append(365.0 * DAY - sum(dts))
+    return dts
+
+
+def test_clm_nmin_matches_the_gold():
+    """SOIL4 (C:N 10, turnover 27.4 y of 365 d) respired to CO2 with N
+    mineralized to NH4+ and emitted as N2O, tracked in Nmin / HRimm /
+    NGASmin, pH 6.5"""
+    prim = ["CO2(aq)", "NH4+", "N2O(aq)", "H+"]
+    imm = ["SOIL4", "Nmin", "HRimm", "NGASmin"]
+    cc = _som_cell(prim, imm, [SD.Pool("SOIL4", 10.0)],
+                   [SD.Reaction("SOIL4", [], turnover_time=27.4 * 365.0 * DAY)],
+                   np.array([1e-40, 1e-40, 1e-40, 10.0 ** -6.5]), [10.0, 1e-40, 1e-40, 1e-40])
+    dts = _nmin_steps()
+    fin = GU.golds()["clm_nmin"]["final"]
+    assert len(dts) == fin["SOLUTION: Transport: Time Steps"]
+    for dt in dts:
+        err, nit, why = cc.rreact(dt)
+        assert err == 0
+    tot = cc.get("total", 4)
+    im = cc.get("immobile", 4)
+    got = {"Total CO2(aq)": tot[0], "Total NH4+": tot[1], "Total N2O(aq)": tot[2],
+           "SOIL4": im[0], "Nmin": im[1], "HRimm": im[2], "NGASmin": im[3]}
+    rel = _gold_rel("clm_nmin", got)
+    print("clm_nmin", rel)
+    for k, e in rel.items():
+        assert e < 1e-12, (k, e)
+
+
+@pytest.mark.parametrize("temp,factor", [(25.0, 1.0), (35.0, 2.0), (15.0, 0.5), (5.0, 0.25),
+                                         (0.0, 2.0 ** -2.5)])
+def test_q10_response_closed_form(temp, factor):
+    """first-order decay of one fixed-C:N pool under the Q10 temperature
+    response (Q10 = 2, reference 25 C; elm_rspfuncs.F90): one backward-Euler
+    step leaves C0 / (1 + k f_T dt), with f_T = 1 at 25 C, Q10 per 10 C and
+    continuous at 0 C where the branch for frozen soil starts"""
+    k, dt, c0 = 1.0e-6, 1800.0, 10.0
+    prim = ["CO2(aq)", "NH4+", "H+"]
+    cc = _som_cell(prim, ["SOIL4"], [SD.Pool("SOIL4", 10.0)],
+                   [SD.Reaction("SOIL4", [], rate_constant=k,
+                                temperature_response_function=SD.TEMPERATURE_Q10, Q10=2.0)],
+                   np.array([1e-6, 1e-6, 1e-7]), [c0])
+    cc.c.temp = temp
+    err, nit, why = cc.rreact(dt)
+    assert err == 0
+    got = cc.get("immobile", 1)[0]
+    want = c0 / (1.0 + k * factor * dt)
+    assert abs(got - want) / want < 1e-13, (temp, got, want)
