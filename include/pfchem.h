@@ -371,6 +371,47 @@ typedef struct pfchem_launch {
 int pfchem_set_launch(pfchem_ctx* ctx, int variant, const pfchem_launch* cfg);
 int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg);
 
+/* Launch-shape autotuner.  No reference counterpart: the reference runs one
+   cell at a time on the CPU.  Times pfchem_step of `variant` (a batched
+   variant) on the caller's cells (device pointers) for the launch shapes
+   the context accepts: the one-lane kernel with 0, 1 and 2 workgroups per
+   CU as cap; every compiled team shape with lanes x rows in
+   [ncomp, max(4 ncomp, 16)] at 256-lane workgroups, 4 per CU; then the
+   three fastest teams (those within 2x of the best shape so far) over
+   workgroup sizes 64/128/256 and 1/2/4/8 workgroups per CU.  With per-cell active counts, each size bucket's team
+   (16x1, 32x1, 64x1) is swept over workgroup size and grid in turn.  Every
+   timing is the best of 3 steps (1 with PFCHEM_TUNE_QUICK) after an untimed
+   one, each from the caller's state as given; the state is left as given.
+   Everything runs on `stream` (NULL: the default stream); the call
+   synchronizes it.
+   The fastest shape whose cells all end with the starting shape's status
+   and free molalities within 1e-6 relative becomes the context's shape for
+   `variant` and is returned in `chosen`.
+   cache_path (optional): JSON cache of choices keyed by device, network
+   content and cell-count bucket (power of two).  An entry for this exact
+   key and bucket is applied without measuring unless flags has
+   PFCHEM_TUNE_FORCE; a measured choice replaces it.  pfchem_create applies
+   the cache named by the environment variable PFCHEM_TUNE_CACHE (exact
+   network first, else the same system-size bucket).  table_path
+   (optional): text table of every candidate's time, difference and
+   outcome.  Allocates device scratch for a copy of the state's output
+   fields. */
+#define PFCHEM_TUNE_FORCE 1
+#define PFCHEM_TUNE_QUICK 2
+int pfchem_autotune(pfchem_ctx* ctx, int variant, int64_t ncell, const pfchem_state* state,
+                    double dt, const char* cache_path, const char* table_path, int flags,
+                    pfchem_launch* chosen, void* stream);
+/* The cache keys of a context: `key` (device, network content) and
+   `bucket` (device, system size rounded up to a power of two, kernel
+   family). */
+int pfchem_tune_key(const pfchem_ctx* ctx, char* key, int key_len, char* bucket,
+                    int bucket_len);
+/* Look up a cache file without a device: returns 1 and the entry's launch
+   shape for (key, else bucket; variant; nearest cell-count bucket), 0 if
+   none, or a negative error. */
+int pfchem_tune_cache_query(const char* cache_path, const char* key, const char* bucket,
+                            int variant, int64_t ncell, pfchem_launch* out);
+
 /* Generate and compile the network-specialized one-lane kernels of
    (net, opt) for gfx950 without a device, e.g. to check a network at set-up
    time (pfchem_step compiles them on the device at its first batched call).

@@ -11,8 +11,12 @@ import numpy as np
 
 ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# the same sources built -O3 with FMA contraction: timing of the CPU
+# baseline only, never the checker
+ORACLE_FAST_SO = os.path.join(ORACLE_DIR, "liboracle_fast.so")
 
 _lib = None
+_fast = None
 
 MAXCOMP, MAXCPLX, MAXMNRL, MAXRXN, SOMDEC_MAX_DOWN = 96, 512, 32, 32, 4
 
@@ -127,13 +131,29 @@ def record_fields(records, name):
     return np.ndarray((records.ncell,), dtype=np.float64, buffer=raw, offset=idx, strides=(size,))
 
 
-def step_batch(net, opt, state, dt, nthreads=0, records=None):
-    """RStep over all cells of a host CellState (in place). Returns #failed."""
+def fast_lib():
+    """the -O3 build of the oracle (CPU-baseline timing only)"""
+    global _fast
+    if _fast is None:
+        if not os.path.exists(ORACLE_FAST_SO):
+            build()
+        L = C.CDLL(ORACLE_FAST_SO)
+        L.orc_step_batch.restype = C.c_int64
+        L.orc_step_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                     C.c_double, C.c_int, C.c_void_p]
+        _fast = L
+    return _fast
+
+
+def step_batch(net, opt, state, dt, nthreads=0, records=None, fast=False):
+    """RStep over all cells of a host CellState (in place). Returns #failed.
+    fast: run the -O3 build (for timing the CPU baseline)."""
     n = net.to_abi()
     o = opt.to_abi()
     s = state.to_abi()
-    return lib().orc_step_batch(C.byref(n), C.byref(o), C.byref(s), state.ncell, dt,
-                                nthreads, records.ptr if records else None)
+    L = fast_lib() if fast else lib()
+    return L.orc_step_batch(C.byref(n), C.byref(o), C.byref(s), state.ncell, dt,
+                            nthreads, records.ptr if records else None)
 
 
 def update_activity_batch(net, opt, state, nthreads=0):

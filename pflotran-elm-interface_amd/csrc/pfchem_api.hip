@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -12,6 +13,7 @@
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
 #include "pfchem_lane.h"
+#include "pfchem_tune.h"
 
 using namespace pfchem;
 
@@ -61,6 +63,10 @@ struct pfchem_ctx {
   LaunchCfg team_default{16, 1, 256, 4};  // team shape of the FP64 rescue pass
   int32_t* status_scratch = nullptr;      // per-cell status when the caller gives none
   int64_t status_cap = 0;
+  // launch-shape cache identity: device architecture and a hash of the
+  // network arrays and options
+  std::string arch = "gfx";
+  uint64_t net_hash = 0;
   std::string err;
 };
 
@@ -418,6 +424,36 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     so[q++] = addd(sdh->inhibition_constant2, nri);
   }
 
+  {
+    // FNV-1a over the packed network, its sizes and the options
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void* p, size_t n) {
+      const unsigned char* b = static_cast<const unsigned char*>(p);
+      for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    mix(P.bytes.data(), P.bytes.size());
+    const int32_t sizes[8] = {naq, nimm, nc, nm, net->ndecay_rxn, net->eqcplx_maxsp,
+                              net->kinmnrl_maxsp, sdh ? sdh->nrxn : -1};
+    mix(sizes, sizeof(sizes));
+    mix(opt, sizeof(*opt));
+    if (sdh) {
+      const double sd_scalars[3] = {sdh->x0eps, sdh->n2o_frac_mineralization,
+                                    sdh->inhibition_nh4_no3};
+      const int32_t sd_ids[10] = {sdh->species_id_co2,    sdh->species_id_nh4,
+                                  sdh->species_id_no3,    sdh->species_id_n2o,
+                                  sdh->species_id_proton, sdh->species_id_hr,
+                                  sdh->species_id_nmin,   sdh->species_id_nimm,
+                                  sdh->species_id_nimp,   sdh->species_id_ngasmin};
+      mix(sd_scalars, sizeof(sd_scalars));
+      mix(sd_ids, sizeof(sd_ids));
+    }
+    ctx->net_hash = h;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+      std::string a = prop.gcnArchName;
+      ctx->arch = a.substr(0, a.find(':'));
+    }
+  }
   e = hipMalloc(&ctx->dbuf, P.bytes.size());
   if (e != hipSuccess) {
     delete ctx;
@@ -557,7 +593,8 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   ctx->opt = *opt;
   ctx->has_somdec = net->somdec != nullptr;
   // default team shape (lanes per cell x rows per lane) by system size;
-  // the autotuner replaces it with a measured choice
+  // pfchem_autotune measures the alternatives, and a cache of its choices
+  // is applied at the end of set-up
   int L = 32, R = 1;
   if (d.ncomp > 32) {
     L = 64;  // one cell per wavefront
@@ -610,9 +647,57 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     pfchem_destroy(ctx);
     return PFCHEM_ERR_NOMEM;
   }
+  // measured launch shapes (pfchem_autotune) from the cache file the
+  // environment names, where they match this device and network
+  if (const char* cache = std::getenv("PFCHEM_TUNE_CACHE"))
+    if (*cache) (void)tune_apply_cache(ctx, cache);
   *out = ctx;
   return PFCHEM_OK;
 }
+
+}  // extern "C"
+
+namespace pfchem {
+
+TuneInfo tune_info(const pfchem_ctx* ctx) {
+  TuneInfo t;
+  const DevNet& d = ctx->dnet;
+  t.naq = ctx->net.naqcomp;
+  t.nimm = ctx->net.nimmobile;
+  t.neqcplx = ctx->net.neqcplx;
+  t.nkinmnrl = ctx->net.nkinmnrl;
+  t.ncomp = d.ncomp;
+  t.variable = d.variable;
+  t.lane_ok = ctx->lane_ok;
+  int p2 = 1;
+  while (p2 < d.ncomp) p2 *= 2;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s/%dcu/aq%d/im%d/cx%d/mn%d/sd%d/var%d/%016llx",
+                ctx->arch.c_str(), ctx->num_cus, t.naq, t.nimm, t.neqcplx, t.nkinmnrl,
+                ctx->has_somdec ? 1 : 0, t.variable ? 1 : 0, (unsigned long long)ctx->net_hash);
+  t.key = buf;
+  std::snprintf(buf, sizeof(buf), "%s/%dcu/n%d/sd%d/var%d/lane%d", ctx->arch.c_str(),
+                ctx->num_cus, p2, ctx->has_somdec ? 1 : 0, t.variable ? 1 : 0,
+                ctx->lane_ok ? 1 : 0);
+  t.bucket = buf;
+  if (t.variable) {
+    for (int b = 0; b < kBuckets && b < kTuneSizeBuckets; ++b) {
+      if (b > 0 && kBucketBound[b - 1] >= d.naq) break;  // no cell that large
+      t.bucket_bound[b] = kBucketBound[b];
+      for (int v = 0; v < 2; ++v) {
+        const LaunchCfg& c = ctx->vlaunch[v][b];
+        t.bucket_launch[v][b] = pfchem_launch{c.lanes_per_cell, c.block_size, c.blocks_per_cu,
+                                              c.rows_per_lane};
+      }
+      t.size_buckets = b + 1;
+    }
+  }
+  return t;
+}
+
+}  // namespace pfchem
+
+extern "C" {
 
 int pfchem_destroy(pfchem_ctx* ctx) {
   if (!ctx) return PFCHEM_OK;
@@ -680,9 +765,10 @@ static int to_dev_state(pfchem_ctx* ctx, const pfchem_state* st, int64_t ncell, 
 }
 
 // Batched step on the network-specialized lane kernel: FP32 (mixed) or FP64
-// pass over every cell; for mixed an FP64 lane pass over the cells the FP32
-// pass handed over; then the FP64 team kernel (pivoting fallback) over the
-// cells still handed over.
+// pass over every cell, then the FP64 team kernel (pivoting) over the cells
+// handed over.  The hand-over pass runs few cells, so its time is the
+// latency of its slowest cell: a team of lanes runs one Newton iteration
+// several times faster than one lane does.
 static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double dt, bool mixed,
                      hipStream_t s) {
   if (!ctx->lane_ok)
@@ -732,16 +818,11 @@ static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double d
   const int bpc = ctx->launch[mixed ? PFCHEM_KERNEL_BATCHED_MIXED : PFCHEM_KERNEL_BATCHED_FP64]
                       .blocks_per_cu;
   hipError_t e;
-  if (mixed) {
-    a.rescue = 0;
-    e = lane_launch(ctx->lane_src, ctx->device, 1, a, ctx->num_cus, bpc, ctx->counter, s);
-    if (e != hipSuccess) return hip_fail(ctx, e, "lane kernel launch (FP32)");
-    a.rescue = 1;
-  } else {
-    a.rescue = 0;
-  }
-  e = lane_launch(ctx->lane_src, ctx->device, 0, a, ctx->num_cus, bpc, ctx->counter, s);
-  if (e != hipSuccess) return hip_fail(ctx, e, "lane kernel launch (FP64)");
+  a.rescue = 0;
+  e = lane_launch(ctx->lane_src, ctx->device, mixed ? 1 : 0, a, ctx->num_cus, bpc, ctx->counter,
+                  s);
+  if (e != hipSuccess)
+    return hip_fail(ctx, e, mixed ? "lane kernel launch (FP32)" : "lane kernel launch (FP64)");
   DevState dr = d;
   dr.status = st;
   int k = 0;

@@ -58,8 +58,29 @@ def load(path: str = LIB_PATH):
     L.pfchem_adjoint.argtypes = [vp, C.c_int64, vp, C.c_double, vp, vp, C.c_int, vp, vp, vp, vp,
                                  vp]
     L.pfchem_prepare_kernels.argtypes = [vp, vp, C.c_char_p]
+    L.pfchem_autotune.argtypes = [vp, C.c_int, C.c_int64, vp, C.c_double, C.c_char_p,
+                                  C.c_char_p, C.c_int, vp, vp]
+    L.pfchem_tune_key.argtypes = [vp, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+    L.pfchem_tune_cache_query.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int,
+                                          C.c_int64, vp]
     _lib = L
     return L
+
+
+def tune_cache_query(cache_path: str, key: str, bucket: str, variant: int,
+                     ncell: int) -> Optional[dict]:
+    """pfchem_tune_cache_query: the launch shape a tuner cache file holds
+    for (key, else bucket; variant; nearest cell-count bucket), or None"""
+    L = load()
+    c = abi.Launch()
+    rc = L.pfchem_tune_cache_query(cache_path.encode(), key.encode(), bucket.encode(), variant,
+                                   ncell, C.byref(c))
+    if rc < 0:
+        raise PfchemError(-rc, "pfchem_tune_cache_query")
+    if rc == 0:
+        return None
+    return dict(lanes_per_cell=c.lanes_per_cell, rows_per_lane=c.rows_per_lane,
+                block_size=c.block_size, blocks_per_cu=c.blocks_per_cu)
 
 
 def prepare_kernels(net: ReactionNetwork, opt: SolverOptions,
@@ -284,6 +305,31 @@ class Context:
                                           f.data_ptr(), status.data_ptr(),
                                           _stream_ptr(stream)), "pfchem_adjoint")
         return grad[:np_], f, status, gc
+
+    def autotune(self, st: DeviceState, dt: float, variant: int = abi.KERNEL_BATCHED_MIXED,
+                 cache_path: Optional[str] = None, table_path: Optional[str] = None,
+                 force: bool = False, quick: bool = False) -> dict:
+        """pfchem_autotune: time the step over the launch shapes the context
+        accepts on these cells, keep the fastest that agrees with the current
+        shape; returns the chosen shape (the state is left as given)"""
+        import torch
+        s = st.to_abi()
+        c = abi.Launch()
+        flags = (1 if force else 0) | (2 if quick else 0)
+        sp = torch.cuda.current_stream().cuda_stream
+        self._check(self.L.pfchem_autotune(self.h, variant, st.ncell, C.byref(s), dt,
+                                           cache_path.encode() if cache_path else None,
+                                           table_path.encode() if table_path else None,
+                                           flags, C.byref(c), sp), "pfchem_autotune")
+        return dict(lanes_per_cell=c.lanes_per_cell, rows_per_lane=c.rows_per_lane,
+                    block_size=c.block_size, blocks_per_cu=c.blocks_per_cu)
+
+    def tune_key(self):
+        """(key, bucket) of this context in a tuner cache"""
+        k = C.create_string_buffer(256)
+        b = C.create_string_buffer(256)
+        self._check(self.L.pfchem_tune_key(self.h, k, 256, b, 256), "pfchem_tune_key")
+        return k.value.decode(), b.value.decode()
 
     def get_launch(self, variant):
         c = abi.Launch()
