@@ -60,6 +60,8 @@ struct pfchem_ctx {
   bool lane_ok = false;
   bool lane_ready = false;
   std::string lane_why;
+  std::string lane_adj_src;  // the adjoint module's source (compiled at first use)
+  bool lane_adj_ready = false;
   LaunchCfg team_default{16, 1, 256, 4};  // team shape of the FP64 rescue pass
   int32_t* status_scratch = nullptr;      // per-cell status when the caller gives none
   int64_t status_cap = 0;
@@ -1195,8 +1197,22 @@ int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
   hipStream_t s = (hipStream_t)stream;
   rc = sens_stage(ctx, w_vf, w_sec, nullptr, s);
   if (rc) return rc;
-  // per-cell gradients and status: caller's or the context's scratch
-  size_t need = 0;
+  // the network-specialized adjoint (one cell per lane) where the one-lane
+  // kernel takes the network; the cells it hands over, and every cell of
+  // other networks, go to the general kernel
+  const bool lane = ctx->lane_ok;
+  if (lane && !ctx->lane_adj_ready) {
+    ctx->lane_adj_src = std::string(kLaneAdjointDefine) + ctx->lane_src;
+    std::string msg;
+    const int crc = lane_compile(ctx->lane_adj_src, ctx->device, msg);
+    if (crc) return fail(ctx, crc, msg);
+    ctx->lane_adj_ready = true;
+  }
+  const size_t nlog =
+      lane ? (size_t)kLaneSensSub * (ctx->dnet.naq + ctx->dnet.nm + 1) * (size_t)ncell : 0;
+  // per-cell gradients and status: caller's or the context's scratch; the
+  // sub-step log of the one-lane adjoint
+  size_t need = nlog * sizeof(double);
   if (!grad_cell) need += (size_t)np * ncell * sizeof(double);
   if (!status) need += (size_t)ncell * sizeof(int32_t);
   if (need > ctx->sens_work_bytes) {
@@ -1208,6 +1224,11 @@ int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
     ctx->sens_work_bytes = need;
   }
   char* wp = (char*)ctx->sens_work;
+  double* lg = nullptr;
+  if (nlog) {
+    lg = (double*)wp;
+    wp += nlog * sizeof(double);
+  }
   double* gc = grad_cell;
   if (!gc) {
     gc = (double*)wp;
@@ -1215,12 +1236,41 @@ int pfchem_adjoint(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, do
   }
   int32_t* st = status ? status : (int32_t*)wp;
   hipError_t e;
+  if (lane) {
+    LaneSensArgs a{};
+    a.ld = d.ld;
+    a.ncell = ncell;
+    a.pri_molal = d.pri_molal;
+    a.total = d.total;
+    a.pri_act_coef = d.pri_act_coef;
+    a.sec_act_coef = d.sec_act_coef;
+    a.ln_act_h2o = d.ln_act_h2o;
+    a.mnrl_volfrac = d.mnrl_volfrac;
+    a.mnrl_area = d.mnrl_area;
+    a.porosity = d.porosity;
+    a.saturation = d.saturation;
+    a.den_kg = d.den_kg;
+    a.volume = d.volume;
+    a.temperature = d.temperature;
+    a.wv = ctx->sens_in;
+    a.ws = ctx->sens_in + ctx->dnet.nm;
+    a.grad = gc;
+    a.ldo = ncell;
+    a.fcell = f_cell;
+    a.status = st;
+    a.log = lg;
+    a.ldl = ncell;
+    a.dt = dt;
+    a.jacobian = jacobian ? 1 : 0;
+    e = lane_adjoint_launch(ctx->lane_adj_src, ctx->device, a, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "adjoint launch");
+  }
   int k = 0;
   DevState* slot = acquire_slot(ctx, d, s, &k, &e);
   if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
   e = launch_sens(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, ctx->sens_in,
                   ctx->sens_in + ctx->dnet.nm, nullptr, 1, jacobian ? 1 : 0, f_cell, gc, ncell,
-                  st, ctx->sens_sum, s);
+                  st, ctx->sens_sum, s, lane ? 1 : 0);
   if (e == hipSuccess) e = release_slot(ctx, k, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(ctx->sens_sum_h, ctx->sens_sum, np * sizeof(double),

@@ -91,6 +91,6 @@ int sens_supported(const DevNet& n, const pfchem_options& opt);
 hipError_t launch_sens(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                        double dt, const double* wv, const double* ws, const double* dp, int mode,
                        int jacobian, double* fcell, double* out, int64_t ldo, int32_t* status,
-                       double* sum, hipStream_t stream);
+                       double* sum, hipStream_t stream, int rescue = 0);
 
 }  // namespace pfchem

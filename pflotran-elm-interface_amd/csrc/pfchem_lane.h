@@ -48,6 +48,38 @@ struct LaneArgs {
 };
 static_assert(sizeof(LaneArgs) == 192, "LaneArgs layout");
 
+// adjoint kernel arguments; the same layout as pflane::SensArgs
+struct LaneSensArgs {
+  long long ld, ncell;
+  const double* pri_molal;
+  const double* total;
+  const double* pri_act_coef;
+  const double* sec_act_coef;
+  const double* ln_act_h2o;
+  const double* mnrl_volfrac;
+  const double* mnrl_area;
+  const double* porosity;
+  const double* saturation;
+  const double* den_kg;
+  const double* volume;
+  const double* temperature;
+  const double* wv;
+  const double* ws;
+  double* grad;
+  long long ldo;
+  double* fcell;
+  int* status;
+  double* log;
+  long long ldl;
+  double dt;
+  int jacobian;
+  int pad;
+};
+static_assert(sizeof(LaneSensArgs) == 192, "LaneSensArgs layout");
+constexpr int kLaneSensSub = 8;  // sub-steps the adjoint kernel records per cell
+// first line of an adjoint module's source (followed by lane_source)
+constexpr char kLaneAdjointDefine[] = "#define PFLANE_ADJOINT 1\n";
+
 // 1 if the lane kernel handles this network and these options (else `why`)
 int lane_supported(const pfchem_network* net, const pfchem_options* opt, std::string& why);
 // full device source: the network as constants followed by the template
@@ -64,5 +96,8 @@ hipError_t lane_launch(const std::string& src, int device, int f32, const LaneAr
                        hipStream_t stream);
 // resident 256-lane workgroups per CU of a compiled kernel (0: not compiled)
 int lane_occupancy(const std::string& src, int device, int f32);
+// launch the adjoint kernel of a compiled source, one cell per lane
+hipError_t lane_adjoint_launch(const std::string& src, int device, const LaneSensArgs& args,
+                               hipStream_t stream);
 
 }  // namespace pfchem

@@ -106,7 +106,7 @@ void rows(const int32_t* ids, const double* st, int n, int ms, std::vector<int>&
 
 struct Module {
   hipModule_t mod = nullptr;
-  hipFunction_t f32 = nullptr, f64 = nullptr;
+  hipFunction_t f32 = nullptr, f64 = nullptr, adj = nullptr;
   int blocks_per_cu32 = 1, blocks_per_cu64 = 1;
 };
 std::mutex g_mu;
@@ -296,11 +296,14 @@ int lane_compile(const std::string& src, int device, std::string& err) {
   Module m;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipModuleLoadData(&m.mod, code.data());
-  if (e == hipSuccess) e = hipModuleGetFunction(&m.f32, m.mod, "pfchem_lane_step_f32");
-  if (e == hipSuccess) e = hipModuleGetFunction(&m.f64, m.mod, "pfchem_lane_step_f64");
-  if (e == hipSuccess)
+  // a step module holds the two step kernels, an adjoint module the adjoint
+  const bool adjoint = src.compare(0, sizeof(kLaneAdjointDefine) - 1, kLaneAdjointDefine) == 0;
+  if (e == hipSuccess && adjoint) e = hipModuleGetFunction(&m.adj, m.mod, "pfchem_lane_adjoint");
+  if (e == hipSuccess && !adjoint) e = hipModuleGetFunction(&m.f32, m.mod, "pfchem_lane_step_f32");
+  if (e == hipSuccess && !adjoint) e = hipModuleGetFunction(&m.f64, m.mod, "pfchem_lane_step_f64");
+  if (e == hipSuccess && !adjoint)
     e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&m.blocks_per_cu32, m.f32, 256, 0);
-  if (e == hipSuccess)
+  if (e == hipSuccess && !adjoint)
     e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&m.blocks_per_cu64, m.f64, 256, 0);
   if (e != hipSuccess) {
     err = std::string("lane kernel load: ") + hipGetErrorString(e);
@@ -367,6 +370,23 @@ hipError_t lane_launch(const std::string& src, int device, int f32, const LaneAr
   void* params[] = {&a};
   return hipModuleLaunchKernel(f32 ? m.f32 : m.f64, (unsigned)blocks, 1, 1, 256, 1, 1, 0, stream,
                                params, nullptr);
+}
+
+hipError_t lane_adjoint_launch(const std::string& src, int device, const LaneSensArgs& args,
+                               hipStream_t stream) {
+  Module m;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_mods.find(std::make_pair(device, src));
+    if (it == g_mods.end()) return hipErrorInvalidHandle;
+    m = it->second;
+  }
+  const int64_t blocks = (args.ncell + 255) / 256;
+  if (blocks < 1) return hipSuccess;
+  LaneSensArgs a = args;
+  void* params[] = {&a};
+  return hipModuleLaunchKernel(m.adj, (unsigned)blocks, 1, 1, 256, 1, 1, 0, stream, params,
+                               nullptr);
 }
 
 int lane_occupancy(const std::string& src, int device, int f32) {

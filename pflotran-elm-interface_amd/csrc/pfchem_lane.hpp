@@ -876,9 +876,599 @@ __device__ __forceinline__ void lane_step(const Args& A) {
 
 }  // namespace pflane
 
+// one module holds either the step kernels or (PFLANE_ADJOINT defined) the
+// adjoint kernel, so a context that never asks for sensitivities does not
+// compile it
+#ifndef PFLANE_ADJOINT
 extern "C" __global__ void __launch_bounds__(256) pfchem_lane_step_f32(pflane::Args a) {
   pflane::lane_step<float>(a);
 }
 extern "C" __global__ void __launch_bounds__(256) pfchem_lane_step_f64(pflane::Args a) {
   pflane::lane_step<double>(a);
 }
+#endif
+
+// ---------------------------------------------------------------------------
+// Adjoint of one reaction step, one cell per lane, FP64.
+//
+// The step is RStep's accepted sub-steps s of size h_s, each the converged
+// RReact system (reaction.F90:3742-4055)
+//   F(m; T0, VF0, p) = psv (T(m; p) - T0) / h + vol sum_q nu_q Im_q(m; p) = 0
+// followed by VF = max(0, VF0 + Im V_m h) (reaction_mineral.F90:1456-1525);
+// the next sub-step starts from T(m), VF.  Objective per cell:
+// f = sum_q wv_q VF_q(end) + sum_k ws_k m_k(end); parameters log10 k_q
+// (rate constants) and log10 K_k (complexes).  The forward pass repeats the
+// step (the FP64 solve of pfchem_lane_step_f64) and records each accepted
+// sub-step (m, VF0, h) in a per-cell log; the reverse pass applies the
+// implicit-function theorem at each recorded solution, one solve per
+// sub-step.  In the log form J diag(m) = jc M with M symmetric
+// (M = diag(m) + S^T diag(m_k) S + sum_q g_q nu_q nu_q^T), so the transposed
+// solve J^T lambda = b is M lambda = diag(m) b / jc with the same L D L^T.
+// The mineral block of M carries rho/1000 in the Newton iteration (the
+// reference's dQK_dmj, reaction_mineral.F90:945) and in the sensitivities
+// only when asked (jacobian = 0); otherwise it is the exact derivative.
+// Scope: aqueous networks without immobile species, TST rates with optional
+// Arrhenius factor, activity coefficients fixed during the step.  A cell the
+// pass cannot take (a pivot below tolerance, activity coefficients that do
+// not follow the charge/size classes) is left with status kEscalate for the
+// general kernel.
+namespace pflane {
+
+constexpr int kSensSub = 8;  // sub-steps one cell may record
+
+struct SensArgs {
+  i64 ld, ncell;
+  const double* pri_molal;
+  const double* total;
+  const double* pri_act_coef;
+  const double* sec_act_coef;
+  const double* ln_act_h2o;
+  const double* mnrl_volfrac;
+  const double* mnrl_area;
+  const double* porosity;
+  const double* saturation;
+  const double* den_kg;
+  const double* volume;
+  const double* temperature;
+  const double* wv;    // [NM] weights of the end-of-step volume fractions
+  const double* ws;    // [NC] weights of the end-of-step secondary molalities
+  double* grad;        // [(NM + NC) * ldo] per-cell gradient
+  i64 ldo;
+  double* fcell;       // [ncell] objective per cell, or null
+  int* status;         // [ncell]
+  double* log;         // [kSensSub * (NAQ + NM + 1) * ldl] sub-step log
+  i64 ldl;
+  double dt;
+  int jacobian;        // 1: exact dF/dm; 0: the reference's mineral block
+  int pad;
+};
+static_assert(sizeof(SensArgs) == 192, "SensArgs layout");
+
+// L D L^T solve of the symmetrically scaled M (diagonal dg, strict lower
+// packed a; both overwritten by the factors): M y = b in place; false if a
+// pivot is below tolerance
+template <int NA>
+__device__ __forceinline__ bool sens_ldlt(double (&dg)[NA], double (&a)[NA > 1 ? NA * (NA - 1) / 2 : 1],
+                                          double (&b)[NA]) {
+  double s[NA], dinv[NA];
+  sfor<0, NA>([&](auto Q) {
+    constexpr int i = decltype(Q)::value;
+    s[i] = __builtin_amdgcn_rsq(dg[i]);
+    dg[i] = dg[i] * s[i] * s[i];
+    b[i] *= s[i];
+  });
+  sfor<1, NA>([&](auto Q) {
+    constexpr int i = decltype(Q)::value;
+    sfor<0, i>([&](auto Q2) {
+      constexpr int j = decltype(Q2)::value;
+      a[tri(i, j)] *= s[i] * s[j];
+    });
+  });
+  bool bad = false;
+  sfor<0, NA>([&](auto P) {
+    constexpr int p = decltype(P)::value;
+    const double dp = dg[p];
+    if (!(dp > 1.0e-13)) bad = true;
+    const double inv = 1.0 / dp;
+    dinv[p] = inv;
+    double l[NA];
+    sfor<p + 1, NA>([&](auto Q) {
+      constexpr int i = decltype(Q)::value;
+      l[i] = a[tri(i, p)] * inv;
+    });
+    sfor<p + 1, NA>([&](auto Q) {
+      constexpr int i = decltype(Q)::value;
+      dg[i] -= l[i] * a[tri(i, p)];
+      sfor<p + 1, i>([&](auto Q2) {
+        constexpr int j = decltype(Q2)::value;
+        a[tri(i, j)] -= l[i] * a[tri(j, p)];
+      });
+    });
+    sfor<p + 1, NA>([&](auto Q) {
+      constexpr int i = decltype(Q)::value;
+      a[tri(i, p)] = l[i];
+    });
+  });
+  sfor<0, NA>([&](auto Q) {
+    constexpr int i = decltype(Q)::value;
+    sfor<0, i>([&](auto Q2) {
+      constexpr int j = decltype(Q2)::value;
+      b[i] -= a[tri(i, j)] * b[j];
+    });
+  });
+  sfor<0, NA>([&](auto Q) {
+    constexpr int i = decltype(Q)::value;
+    b[i] *= dinv[i];
+  });
+  sfor<0, NA>([&](auto Q) {
+    constexpr int i = NA - 1 - decltype(Q)::value;
+    sfor<i + 1, NA>([&](auto Q2) {
+      constexpr int j = decltype(Q2)::value;
+      b[i] -= a[tri(j, i)] * b[j];
+    });
+  });
+  sfor<0, NA>([&](auto Q) {
+    constexpr int i = decltype(Q)::value;
+    b[i] *= s[i];
+    if (!(b[i] == b[i])) bad = true;
+  });
+  return !bad;
+}
+
+__device__ inline void lane_adjoint(const SensArgs& A) {
+  constexpr int NAQ = Net::NAQ, NC = Net::NC, NM = Net::NM;
+  constexpr int NA1 = NAQ > 0 ? NAQ : 1, NM1 = NM > 0 ? NM : 1;
+  constexpr int NT = NA1 > 1 ? NA1 * (NA1 - 1) / 2 : 1;
+  constexpr int NG = Net::NGCLS, NG1 = NG > 0 ? NG : 1;
+  constexpr int NF = NAQ + NM + 1;  // log fields per sub-step
+  const i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= A.ncell) return;
+  const i64 ld = A.ld, ldo = A.ldo, ldl = A.ldl;
+  if constexpr (Net::NIMM > 0 || Net::ACT_FREQ == kFreqNewtonIter || !Net::FULL_GEOCHEM) {
+    A.status[c] = kEscalate;  // outside this pass's scope
+    return;
+  } else {
+    // ---------------- cell constants ----------------
+    const double por = A.porosity[c], sat = A.saturation[c];
+    const double den = A.den_kg[c], vol = A.volume[c];
+    const double temp = A.temperature ? A.temperature[c] : 25.0;
+    const double tk = temp + 273.15, lnT = pf_log(tk);
+    const double ln_aw = (Net::ACT_H2O && A.ln_act_h2o) ? A.ln_act_h2o[c] : 0.0;
+    const double dkl = den * 1.0e-3, psv = por * sat * 1000.0 * vol;
+    auto zero_grad = [&]() {
+      sfor<0, NM + NC>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        A.grad[q * ldo + c] = 0.0;
+      });
+    };
+    if (sat < kRtMinSat) {  // dry: no sensitivity
+      zero_grad();
+      A.status[c] = 4;  // PFCHEM_CELL_SENS_UNSUPPORTED
+      return;
+    }
+    double lng[NA1], gi[NG1], area[NM1], vf[NM1];
+    sfor<0, NAQ>([&](auto Q) {
+      constexpr int i = decltype(Q)::value;
+      lng[i] = pf_log(A.pri_act_coef[i * ld + c]);
+    });
+    bool cons = true;
+    {
+      double g0[NG1];
+      sfor<0, NC>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int q = Net::CPLX_GCLS[k];
+        const double g = A.sec_act_coef[k * ld + c];
+        if constexpr (q < 0) {
+          cons = cons && g == 1.0;
+        } else if constexpr (Net::CPLX_GFIRST[k]) {
+          g0[q] = g;
+          gi[q] = 1.0 / g;
+        } else {
+          cons = cons && g == g0[q];
+        }
+      });
+    }
+    if (!cons) {
+      A.status[c] = kEscalate;
+      return;
+    }
+    sfor<0, NM>([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      vf[m] = A.mnrl_volfrac[m * ld + c];
+      area[m] = A.mnrl_area[m * ld + c];
+    });
+    // secondary molality of complex K at ln activities lna
+    auto cplx_m = [&](auto K, const double(&lna)[NA1]) {
+      constexpr int k = decltype(K)::value;
+      double lnQK = cplx_lnk<k>(lnT, tk);
+      if constexpr (Net::CPLX_H2O[k] != 0.0) lnQK += Net::CPLX_H2O[k] * ln_aw;
+      sfor<0, Net::CPLX_N[k]>([&](auto Q) {
+        constexpr int a = decltype(Q)::value;
+        lnQK += Net::CPLX_NU[k][a] * lna[Net::CPLX_ID[k][a]];
+      });
+      double m = pf_exp(lnQK);
+      if constexpr (Net::CPLX_GCLS[k] >= 0) m *= gi[Net::CPLX_GCLS[k]];
+      return m;
+    };
+    // rate of mineral M at lna with volume fraction v: Im, dIm/dlnQK
+    auto mnrl_r = [&](auto Mq, const double(&lna)[NA1], double v, double& Im, double& dI) {
+      constexpr int mi = decltype(Mq)::value;
+      double lnQK = mnrl_lnk<mi>(lnT, tk);
+      if constexpr (Net::MNRL_H2O[mi] != 0.0) lnQK += Net::MNRL_H2O[mi] * ln_aw;
+      sfor<0, Net::MNRL_N[mi]>([&](auto Q) {
+        constexpr int a = decltype(Q)::value;
+        lnQK += Net::MNRL_NU[mi][a] * lna[Net::MNRL_ID[mi][a]];
+      });
+      Im = 0.0;
+      dI = 0.0;
+      if (area[mi] == 0.0) return false;
+      const double QK = pf_exp(lnQK);
+      const MRate r = mineral_rate<mi>(lnQK, QK, v, area[mi], temp);
+      if (!r.active) return false;
+      Im = r.Im;
+      dI = r.dIm * QK;
+      return true;
+    };
+    // the system at molalities x: totals (mol/L), residual, and M's diagonal
+    // D / strict lower S (in units of jc) with the mineral block times mf
+    auto system = [&](const double(&x)[NA1], const double(&T0)[NA1], const double(&v0)[NM1],
+                      double h, double mf, double(&T)[NA1], double(&res)[NA1], double(&D)[NA1],
+                      double(&S)[NT]) {
+      double lna[NA1], tot[NA1];
+      const double jc = psv * dkl / h;
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        lna[i] = pf_log(x[i]) + lng[i];
+        tot[i] = 0.0;
+        D[i] = x[i];
+      });
+#pragma unroll
+      for (int q = 0; q < NT; ++q) S[q] = 0.0;
+      sfor<0, NC>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        constexpr int ns = Net::CPLX_N[k];
+        const double m = cplx_m(K, lna);
+        sfor<0, ns>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          constexpr int ia = Net::CPLX_ID[k][a];
+          constexpr double na = Net::CPLX_NU[k][a];
+          tot[ia] += na * m;
+          D[ia] += (na * na) * m;
+          sfor<a + 1, ns>([&](auto Q2) {
+            constexpr int b = decltype(Q2)::value;
+            constexpr int ib = Net::CPLX_ID[k][b];
+            constexpr double nab = na * Net::CPLX_NU[k][b];
+            if constexpr (ia == ib) D[ia] += (2.0 * nab) * m;
+            else S[tri(ia, ib)] += nab * m;
+          });
+        });
+      });
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        T[i] = (x[i] + tot[i]) * dkl;
+        res[i] = (psv * T[i] - psv * T0[i]) / h;
+      });
+      sfor<0, NM>([&](auto Mq) {
+        constexpr int mi = decltype(Mq)::value;
+        constexpr int ns = Net::MNRL_N[mi];
+        double Im, dI;
+        if (!mnrl_r(Mq, lna, v0[mi], Im, dI)) return;
+        const double g = dI * vol * mf / jc;
+        sfor<0, ns>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          constexpr int ia = Net::MNRL_ID[mi][a];
+          constexpr double na = Net::MNRL_NU[mi][a];
+          res[ia] += na * Im * vol;
+          D[ia] += (na * na) * g;
+          sfor<a + 1, ns>([&](auto Q2) {
+            constexpr int b = decltype(Q2)::value;
+            constexpr int ib = Net::MNRL_ID[mi][b];
+            constexpr double nab = na * Net::MNRL_NU[mi][b];
+            if constexpr (ia == ib) D[ia] += (2.0 * nab) * g;
+            else S[tri(ia, ib)] += nab * g;
+          });
+        });
+      });
+    };
+
+    // ---------------- forward: RStep with the sub-step log ----------------
+    double x[NA1], T0[NA1], guess[NA1], T[NA1];
+    sfor<0, NAQ>([&](auto Q) {
+      constexpr int i = decltype(Q)::value;
+      x[i] = A.pri_molal[i * ld + c];
+      T0[i] = A.total[i * ld + c];
+      if (T0[i] <= 1.0e-40) T0[i] = 1.0e-40;  // reaction.F90:3617-3631
+      guess[i] = Net::TOTAL_AS_GUESS ? T0[i] : x[i];
+    });
+    int nlog = 0, st = 0, nconst = 0, ncuts = 0, attempts = 0;
+    double cum = 0.0, tdt = A.dt;
+    bool handover = false;
+    while (cum < A.dt) {
+      if (++attempts > kMaxAttempts) {
+        st = kMaxIter;
+        break;
+      }
+      double r0 = 0.0;
+      int why = -1;
+#pragma unroll
+      for (int i = 0; i < NAQ; ++i) x[i] = guess[i];
+      for (int it = 1; why < 0; ++it) {
+        double res[NA1], D[NA1], S[NT];
+        system(x, T0, vf, tdt, dkl, T, res, D, S);
+        double mx = 0.0, nr2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < NAQ; ++i) {
+          mx = fmax(mx, fabs(res[i]));
+          nr2 += res[i] * res[i];
+        }
+        const double nr = sqrt(nr2);
+        if (it == 1) r0 = nr;
+        if (it > Net::MAX_IT) {
+          why = kMaxIter;
+          break;
+        }
+        if (mx < Net::TOL_ABS || nr / r0 < Net::TOL_REL) {
+          why = kOk;
+          break;
+        }
+        const double jc = psv * dkl / tdt;
+        double delta[NA1];
+#pragma unroll
+        for (int i = 0; i < NAQ; ++i) delta[i] = res[i] / jc;
+        if (!sens_ldlt<NA1>(D, S, delta)) {
+          handover = true;
+          break;
+        }
+        if (!Net::LOG_FORM) {
+#pragma unroll
+          for (int i = 0; i < NAQ; ++i) delta[i] *= x[i];
+        }
+        double latest[NA1];
+        if constexpr (Net::LOG_FORM) {
+#pragma unroll
+          for (int i = 0; i < NAQ; ++i) {
+            const double u = copysign(1.0, delta[i]) * fmin(fabs(delta[i]), Net::MAX_DLNC);
+            latest[i] = x[i] * pf_exp(-u);
+          }
+        } else {
+          double ratio = 1.0e308;
+#pragma unroll
+          for (int i = 0; i < NAQ; ++i)
+            if (x[i] <= delta[i]) ratio = fmin(ratio, fabs(x[i] / delta[i]));
+#pragma unroll
+          for (int i = 0; i < NAQ; ++i)
+            latest[i] = x[i] - (ratio < 1.0 ? delta[i] * ratio * 0.99 : delta[i]);
+        }
+        double rc = -1.0;
+#pragma unroll
+        for (int i = 0; i < NAQ; ++i) {
+          const double q = fabs((latest[i] - x[i]) / x[i]);
+          if (q == q) rc = fmax(rc, q);
+        }
+        if (rc >= 0.0 && rc < Net::TOL_CHANGE) {
+          why = kOk;
+          break;
+        }
+#pragma unroll
+        for (int i = 0; i < NAQ; ++i) x[i] = latest[i];
+      }
+      if (handover) break;
+      if (why != kOk) {
+        // maximum iterations: the totals stay those of the sub-step's start
+        if (++ncuts > Net::MAX_CUTS) {
+          st = kMaxIter;
+          break;
+        }
+        tdt *= 0.5;
+        nconst = 0;
+        continue;
+      }
+      if (nlog == kSensSub) {
+        st = 5;  // PFCHEM_CELL_TOO_MANY_SUBSTEPS
+        break;
+      }
+      double* lg = A.log + (i64)nlog * NF * ldl + c;
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        lg[i * ldl] = x[i];
+      });
+      // kinetic update with the rates at the sub-step's solution
+      {
+        double lna[NA1];
+        sfor<0, NAQ>([&](auto Q) {
+          constexpr int i = decltype(Q)::value;
+          lna[i] = pf_log(x[i]) + lng[i];
+        });
+        sfor<0, NM>([&](auto Mq) {
+          constexpr int mi = decltype(Mq)::value;
+          lg[(NAQ + mi) * ldl] = vf[mi];
+          double Im, dI;
+          mnrl_r(Mq, lna, vf[mi], Im, dI);
+          double v = vf[mi] + Im * Net::MNRL_VM[mi] * tdt;
+          vf[mi] = v < 0.0 ? 0.0 : v;
+        });
+      }
+      lg[(NAQ + NM) * ldl] = tdt;
+      ++nlog;
+#pragma unroll
+      for (int i = 0; i < NAQ; ++i) {
+        T0[i] = T[i];
+        guess[i] = x[i];
+      }
+      cum += tdt;
+      if (++nconst >= 4) {
+        --ncuts;
+        tdt = fmin(2.0 * tdt, A.dt - cum);
+      }
+    }
+    if (handover) {
+      A.status[c] = kEscalate;
+      return;
+    }
+    if (st) {
+      zero_grad();
+      A.status[c] = st;
+      return;
+    }
+
+    // ---------------- objective and its explicit terms ----------------
+    double gm[NA1], gT[NA1], gV[NM1];
+    {
+      double lna[NA1];
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        lna[i] = pf_log(x[i]) + lng[i];
+        gm[i] = 0.0;
+        gT[i] = 0.0;
+      });
+      double f = 0.0;
+      sfor<0, NM>([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        gV[m] = A.wv[m];
+        f += gV[m] * vf[m];
+      });
+      sfor<0, NC>([&](auto K) {
+        constexpr int k = decltype(K)::value;
+        const double wk = A.ws[k];
+        if (wk == 0.0) return;
+        const double mk = cplx_m(K, lna);
+        f += wk * mk;
+        sfor<0, Net::CPLX_N[k]>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          constexpr int ia = Net::CPLX_ID[k][a];
+          gm[ia] += wk * mk * Net::CPLX_NU[k][a] / x[ia];
+        });
+      });
+      if (A.fcell) A.fcell[c] = f;
+    }
+
+    // ---------------- reverse sweep over the recorded sub-steps ----------------
+    const double mf = A.jacobian ? 1.0 : dkl;
+    for (int k = nlog - 1; k >= 0; --k) {
+      const bool first = k == nlog - 1;
+      const double* lg = A.log + (i64)k * NF * ldl + c;
+      double m[NA1], v0[NM1], lna[NA1];
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        m[i] = lg[i * ldl];
+        lna[i] = pf_log(m[i]) + lng[i];
+      });
+      sfor<0, NM>([&](auto M) {
+        constexpr int mi = decltype(M)::value;
+        v0[mi] = lg[(NAQ + mi) * ldl];
+      });
+      const double h = lg[(NAQ + NM) * ldl];
+      const double jc = psv * dkl / h;
+      // kinetic-state update of this sub-step
+      double gkin[NM1], Imq[NM1];
+      bool actq[NM1];
+      sfor<0, NM>([&](auto Mq) {
+        constexpr int mi = decltype(Mq)::value;
+        double Im, dI;
+        actq[mi] = mnrl_r(Mq, lna, v0[mi], Im, dI);
+        Imq[mi] = Im;
+        gkin[mi] = 0.0;
+        if (v0[mi] + Im * Net::MNRL_VM[mi] * h < 0.0) gV[mi] = 0.0;
+        if (gV[mi] == 0.0 || !actq[mi]) return;
+        gkin[mi] = gV[mi] * kLogToLn * Im * Net::MNRL_VM[mi] * h;
+        sfor<0, Net::MNRL_N[mi]>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          constexpr int ia = Net::MNRL_ID[mi][a];
+          gm[ia] += gV[mi] * Net::MNRL_VM[mi] * h * dI * Net::MNRL_NU[mi][a] / m[ia];
+        });
+      });
+      // the sub-step's end totals feed the next sub-step's accumulation:
+      // gm_j += dkl (gT_j + sum_k nu_jk m_k (nu_k . gT) / m_j)
+      double D[NA1], S[NT], Tt[NA1], rs[NA1];
+      {
+        double T0d[NA1];
+#pragma unroll
+        for (int i = 0; i < NAQ; ++i) T0d[i] = 0.0;
+        system(m, T0d, v0, h, mf, Tt, rs, D, S);
+      }
+      double gt[NA1];
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        gt[i] = gT[i] * m[i];
+      });
+      sfor<0, NC>([&](auto K) {
+        constexpr int kk = decltype(K)::value;
+        constexpr int ns = Net::CPLX_N[kk];
+        double dotT = 0.0;
+        sfor<0, ns>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          dotT += Net::CPLX_NU[kk][a] * gT[Net::CPLX_ID[kk][a]];
+        });
+        if (dotT == 0.0) return;
+        const double mk = cplx_m(K, lna);
+        sfor<0, ns>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          gt[Net::CPLX_ID[kk][a]] += Net::CPLX_NU[kk][a] * mk * dotT;
+        });
+      });
+      // J^T lambda = gm  <=>  M lambda = diag(m) gm / jc
+      double lam[NA1];
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        gm[i] += dkl * gt[i] / m[i];
+        lam[i] = m[i] * gm[i] / jc;
+      });
+      if (!sens_ldlt<NA1>(D, S, lam)) {
+        A.status[c] = kEscalate;
+        return;
+      }
+      // log K of the complexes: through the sub-step's totals (gT) and
+      // through the residual (lambda); at the end of the step also the
+      // secondary molality itself
+      sfor<0, NC>([&](auto K) {
+        constexpr int kk = decltype(K)::value;
+        constexpr int ns = Net::CPLX_N[kk];
+        double accT = 0.0, accL = 0.0;
+        sfor<0, ns>([&](auto Q) {
+          constexpr int a = decltype(Q)::value;
+          constexpr int ia = Net::CPLX_ID[kk][a];
+          accT += gT[ia] * Net::CPLX_NU[kk][a];
+          accL += lam[ia] * Net::CPLX_NU[kk][a];
+        });
+        const double wk = first ? A.ws[kk] : 0.0;
+        double* o = A.grad + (i64)(NM + kk) * ldo + c;
+        if (accT == 0.0 && accL == 0.0 && wk == 0.0) {
+          if (first) *o = 0.0;
+          return;
+        }
+        const double mk = cplx_m(K, lna);
+        const double v = (accT - psv / h * accL) * (-kLogToLn * mk * dkl) + wk * (-kLogToLn * mk);
+        *o = first ? v : *o + v;
+      });
+      // rate constants: the rate scales with k
+      sfor<0, NM>([&](auto Mq) {
+        constexpr int mi = decltype(Mq)::value;
+        double v = gkin[mi];
+        if (actq[mi]) {
+          double acc = 0.0;
+          sfor<0, Net::MNRL_N[mi]>([&](auto Q) {
+            constexpr int a = decltype(Q)::value;
+            acc += lam[Net::MNRL_ID[mi][a]] * Net::MNRL_NU[mi][a];
+          });
+          v -= acc * vol * kLogToLn * Imq[mi];
+        }
+        double* o = A.grad + (i64)mi * ldo + c;
+        *o = first ? v : *o + v;
+      });
+      sfor<0, NAQ>([&](auto Q) {
+        constexpr int i = decltype(Q)::value;
+        gT[i] = lam[i] * psv / h;
+        gm[i] = 0.0;
+      });
+    }
+    A.status[c] = 0;
+  }
+}
+
+}  // namespace pflane
+
+#ifdef PFLANE_ADJOINT
+extern "C" __global__ void __launch_bounds__(256) pfchem_lane_adjoint(pflane::SensArgs a) {
+  pflane::lane_adjoint(a);
+}
+#endif

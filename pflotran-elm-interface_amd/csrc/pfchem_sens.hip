@@ -230,11 +230,13 @@ __global__ void __launch_bounds__(64)
 sens_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
             const double* __restrict__ wv, const double* __restrict__ ws,
             const double* __restrict__ dp, int mode, int jacobian, double* __restrict__ fcell,
-            double* __restrict__ out, int64_t ldo, int32_t* __restrict__ status) {
+            double* __restrict__ out, int64_t ldo, int32_t* __restrict__ status, int rescue) {
   const DevNet& n = *np;
   const DevState& s = *sp;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
+  // rescue: only the cells an earlier pass handed over
+  if (rescue && gld(status + c) != kTeamEscalate) return;
   const int naq = n.naq, nm = n.nm, nc = n.nc, npar = nm + nc;
   const int64_t ld = s.ld;
   SensCell<MAXN> w;
@@ -533,12 +535,12 @@ int sens_supported(const DevNet& n, const pfchem_options& opt) {
 hipError_t launch_sens(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                        double dt, const double* wv, const double* ws, const double* dp, int mode,
                        int jacobian, double* fcell, double* out, int64_t ldo, int32_t* status,
-                       double* sum, hipStream_t stream) {
+                       double* sum, hipStream_t stream, int rescue) {
   const int block = 64;
   const dim3 grid((unsigned)((ncell + block - 1) / block));
 #define PFCHEM_SENS(MAXN)                                                                   \
   sens_kernel<MAXN><<<grid, dim3(block), 0, stream>>>(dn, ds, ncell, dt, wv, ws, dp, mode, \
-                                                      jacobian, fcell, out, ldo, status)
+                                                      jacobian, fcell, out, ldo, status, rescue)
   if (n.naq <= 4)
     PFCHEM_SENS(4);
   else if (n.naq <= 8)

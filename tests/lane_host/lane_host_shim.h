@@ -20,6 +20,7 @@ struct PfDim3 {
 };
 static PfDim3 threadIdx = {0, 0, 0};
 static PfDim3 blockIdx = {0, 0, 0};
+static PfDim3 blockDim = {1, 1, 1};
 
 using std::copysign;
 using std::exp;

@@ -57,3 +57,46 @@ def run_host(exe, state, dt, f32=False, escalate=True):
                 a = out.arrays[name]
                 a[...] = np.fromfile(p, dtype=a.dtype).reshape(a.shape)
     return out
+
+
+ADJ_FIELDS = ("pri_molal", "total", "pri_act_coef", "sec_act_coef", "ln_act_h2o", "mnrl_volfrac",
+              "mnrl_area", "porosity", "saturation", "den_kg", "volume", "temperature")
+
+
+def build_host_adjoint(net, opt, workdir):
+    """the generated adjoint kernel (PFLANE_ADJOINT) compiled for the host"""
+    src = os.path.join(workdir, "lane_dev.hip")
+    rc = lib.prepare_kernels(net, opt, src)
+    assert rc == 0, "one-lane kernel does not take this network"
+    with open(src) as f:
+        text = f.read()
+    cpp = os.path.join(workdir, "lane_adj.cpp")
+    with open(cpp, "w") as f:
+        f.write("#define PFLANE_ADJOINT 1\n")
+        f.write(text)
+        f.write('\n#include "lane_host_adjoint.cpp"\n')
+    exe = os.path.join(workdir, "lane_host_adjoint")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-w", "-I", HOST_DIR,
+                    "-include", os.path.join(HOST_DIR, "lane_host_shim.h"), cpp, "-o", exe],
+                   check=True)
+    return exe
+
+
+def run_host_adjoint(exe, net, state, dt, wv, ws, exact=True):
+    """(per-cell gradient [npar, ncell], f per cell, status per cell)"""
+    npar = net.nkinmnrl + net.neqcplx
+    nf = net.naqcomp + net.nkinmnrl + 1
+    n = state.ncell
+    with tempfile.TemporaryDirectory() as d:
+        for name in ADJ_FIELDS:
+            a = state.arrays.get(name)
+            if a is not None and a.shape[0] > 0:
+                np.ascontiguousarray(a).tofile(os.path.join(d, name + ".bin"))
+        np.ascontiguousarray(wv, dtype=np.float64).tofile(os.path.join(d, "wv.bin"))
+        np.ascontiguousarray(ws, dtype=np.float64).tofile(os.path.join(d, "ws.bin"))
+        subprocess.run([exe, d, str(n), str(state.ld), repr(float(dt)), "1" if exact else "0",
+                        str(npar), str(nf)], check=True)
+        g = np.fromfile(os.path.join(d, "grad.out"), dtype=np.float64).reshape(max(npar, 1), n)
+        f = np.fromfile(os.path.join(d, "fcell.out"), dtype=np.float64)
+        st = np.fromfile(os.path.join(d, "status.out"), dtype=np.int32)
+    return g[:npar], f, st

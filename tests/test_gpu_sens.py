@@ -115,3 +115,27 @@ def test_adjoint_vs_central_differences():
     # sensitivities by a visible amount only through the mineral terms
     g_ref, _, _, _ = ctx.adjoint(DeviceState(sub), wl.dt, wv, ws, exact=False)
     assert np.all(np.abs(g_ref - g) <= 1e-2 * np.abs(g) + 1e-30)
+
+
+def test_adjoint_hand_over_to_the_general_kernel():
+    """cells whose secondary activity coefficients do not follow the
+    charge / ion-size classes (here every 5th cell, one complex's
+    coefficient changed) are handed from the one-lane adjoint to the general
+    kernel; every cell still matches the oracle's adjoint"""
+    from pfchem.lib import Context, DeviceState
+    wl, wv, ws = _setup("random15", 1024)
+    st0 = wl.state.copy()
+    n = st0.ncell
+    net = wl.net
+    k = int(np.flatnonzero(np.abs(net.eqcplx_Z) > 0)[0])
+    st0.arrays["sec_act_coef"][k, 0:n:5] *= 1.01
+    print("case sens hand-over", flush=True)
+    g_ref, f_ref, st_ref = O.adjoint_batch(net, wl.opt, st0, wl.dt, wv, ws)
+    ok = st_ref == 0
+    ctx = Context(net, wl.opt)
+    g, f, st, gc = ctx.adjoint(DeviceState(st0), wl.dt, wv, ws, per_cell=True)
+    st = st.cpu().numpy()
+    np.testing.assert_array_equal(st == 0, ok)
+    np.testing.assert_allclose(f.cpu().numpy()[ok], f_ref[ok], rtol=1e-8)
+    scale = np.abs(g_ref).max()
+    np.testing.assert_allclose(g / scale, g_ref / scale, atol=1e-8)

@@ -135,3 +135,32 @@ def test_nonisothermal_network():
 ], ids=["temkin+scale", "power", "limiter+threshold+irreversible"])
 def test_mineral_option_networks(opts):
     _run_pin_case(PU.mineral_options_workload(256, **opts), "options")
+
+
+# the network-specialized adjoint (PFLANE_ADJOINT module), host build, against
+# the oracle's adjoint (implicit-function theorem through the recorded
+# sub-steps); bar 1e-10 of the largest gradient entry (measured ~1e-15)
+@pytest.mark.parametrize("which", ["calcite", "random15"])
+@pytest.mark.parametrize("exact", [True, False], ids=["exact", "reference-jacobian"])
+def test_adjoint_host(which, exact, tmp_path):
+    import oracle_lib as O
+    if which == "calcite":
+        wl, co2 = workloads.calcite_workload(1000), 2
+    else:
+        wl, co2 = workloads.random_network_workload(ncell=512), 0
+    net = wl.net
+    wv = np.ones(net.nkinmnrl)
+    ws = np.zeros(net.neqcplx)
+    ws[co2] = 1.0
+    g_ref, f_ref, st_ref = O.adjoint_batch(net, wl.opt, wl.state, wl.dt, wv, ws, exact)
+    exe = LU.build_host_adjoint(net, wl.opt, str(tmp_path))
+    g, f, st = LU.run_host_adjoint(exe, net, wl.state, wl.dt, wv, ws, exact)
+    np.testing.assert_array_equal(st, st_ref)
+    ok = st == 0
+    assert ok.mean() > 0.95
+    np.testing.assert_allclose(f[ok], f_ref[ok], rtol=1e-12)
+    scale = np.abs(g_ref).max()
+    err = np.abs(g[:, ok].sum(axis=1) - g_ref).max() / scale
+    print(which, exact, "adjoint host vs oracle", err)
+    assert err < 1e-10
+
