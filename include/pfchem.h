@@ -155,6 +155,27 @@ typedef struct pfchem_network {
 
   /* optional SomDec decomposition sandbox (reaction_sandbox_somdec.F90) */
   const pfchem_somdec* somdec;
+
+  /* kinetic-mineral rate prefactors (PREFACTOR blocks, reaction_mineral.F90:
+     820-870; mineral%kinmnrl_num_prefactors ...).  A mineral m with
+     kinmnrl_num_prefactors[m] > 0 replaces its rate constant and activation
+     energy by the sum over its prefactors p of
+       pref_rate[m][p] * Arrhenius(pref_activation_energy[m][p]) *
+       prod_s a_s^alpha[m][p][s] / (1 + atten_coef[m][p][s] a_s^beta[m][p][s])
+     over the prefactor's species s (0-based primary ids; the reference's
+     secondary-species prefactors are not supported).  NULL
+     kinmnrl_num_prefactors: no mineral has prefactors.  The one-lane
+     kernel and the sensitivity entry points do not take such networks. */
+  int32_t kinmnrl_max_pref;        /* prefactors per mineral (array stride)   */
+  int32_t kinmnrl_max_pref_spec;   /* species per prefactor (array stride)    */
+  const int32_t* kinmnrl_num_prefactors;  /* [nkinmnrl]                       */
+  const int32_t* kinmnrl_prefactor_id;    /* [nkinmnrl][max_pref][max_pref_spec+1];
+                                             [m][p][0] = species count        */
+  const double* kinmnrl_pref_alpha;       /* [nkinmnrl][max_pref][max_pref_spec] */
+  const double* kinmnrl_pref_beta;        /* same                             */
+  const double* kinmnrl_pref_atten_coef;  /* same                             */
+  const double* kinmnrl_pref_rate;        /* [nkinmnrl][max_pref] mol/m^2/s   */
+  const double* kinmnrl_pref_activation_energy; /* [nkinmnrl][max_pref] J/mol, or NULL */
 } pfchem_network;
 
 /* solver options: the reaction_rt_type control fields */

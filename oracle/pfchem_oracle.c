@@ -302,8 +302,9 @@ void orc_accumulation_derivative(const pfchem_network* net, const orc_cell* c,
   }
 }
 
-/* RKineticMineral reaction_mineral.F90:647-1078 (default path, no
-   prefactors / solid solution) */
+/* RKineticMineral reaction_mineral.F90:647-1078 (no solid solution;
+   prefactor species primary only) */
+#define ORC_MAXPREF 8
 static void kinetic_mineral(const pfchem_network* net, orc_cell* c, double* res,
                             double* jac, int compute_derivative) {
   const int naq = net->naqcomp, n = ncomp_of(net), ms = net->kinmnrl_maxsp;
@@ -337,6 +338,9 @@ static void kinetic_mineral(const pfchem_network* net, orc_cell* c, double* res,
     }
     double sgn = copysign(1.0, aff);
     double Im, Im_const, sum_prefactor_rate;
+    const int npref = net->kinmnrl_num_prefactors ? net->kinmnrl_num_prefactors[m] : 0;
+    const int mp = net->kinmnrl_max_pref, msp = net->kinmnrl_max_pref_spec;
+    double prefactor[ORC_MAXPREF];
     if (c->mnrl_volfrac[m] > 0.0 || sgn < 0.0) {
       if (net->kinmnrl_irreversible && net->kinmnrl_irreversible[m] == 1 && sgn < 0.0)
         continue;
@@ -345,12 +349,41 @@ static void kinetic_mineral(const pfchem_network* net, orc_cell* c, double* res,
       }
       double limiter = net->kinmnrl_rate_limiter ? net->kinmnrl_rate_limiter[m] : 0.0;
       if (limiter > 0.0) aff = aff / (1.0 + (1.0 - aff) / limiter);
-      double arrhenius = 1.0;
-      double Ea = net->kinmnrl_activation_energy ? net->kinmnrl_activation_energy[m] : 0.0;
-      if (Ea > 0.0)
-        arrhenius = exp(Ea / IDEAL_GAS_CONSTANT *
-                        (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
-      sum_prefactor_rate = net->kinmnrl_rate_constant[m] * arrhenius;
+      if (npref > 0) {
+        /* parallel prefactor rates (:820-860) */
+        sum_prefactor_rate = 0.0;
+        for (int p = 0; p < npref; ++p) {
+          const int32_t* pid = net->kinmnrl_prefactor_id + ((size_t)m * mp + p) * (msp + 1);
+          double ln_prefactor = 0.0;
+          for (int q = 0; q < pid[0]; ++q) {
+            const size_t k = ((size_t)m * mp + p) * msp + q;
+            const double ln_spec_act = ln_act[pid[1 + q]];
+            const double ln_numerator = net->kinmnrl_pref_alpha[k] * ln_spec_act;
+            const double ln_denominator =
+                log(1.0 + exp(log(net->kinmnrl_pref_atten_coef[k]) +
+                              net->kinmnrl_pref_beta[k] * ln_spec_act));
+            ln_prefactor = ln_prefactor + ln_numerator;
+            ln_prefactor = ln_prefactor - ln_denominator;
+          }
+          prefactor[p] = exp(ln_prefactor);
+          double arrhenius = 1.0;
+          const double Ea = net->kinmnrl_pref_activation_energy
+                                ? net->kinmnrl_pref_activation_energy[(size_t)m * mp + p]
+                                : 0.0;
+          if (Ea > 0.0)
+            arrhenius = exp(Ea / IDEAL_GAS_CONSTANT *
+                            (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
+          sum_prefactor_rate = sum_prefactor_rate +
+                               prefactor[p] * net->kinmnrl_pref_rate[(size_t)m * mp + p] * arrhenius;
+        }
+      } else {
+        double arrhenius = 1.0;
+        double Ea = net->kinmnrl_activation_energy ? net->kinmnrl_activation_energy[m] : 0.0;
+        if (Ea > 0.0)
+          arrhenius = exp(Ea / IDEAL_GAS_CONSTANT *
+                          (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
+        sum_prefactor_rate = net->kinmnrl_rate_constant[m] * arrhenius;
+      }
       Im_const = -c->mnrl_area[m];
       if (mscale) Im_const = Im_const / mscale[m];
       if (net->kinmnrl_affinity_power)
@@ -397,6 +430,42 @@ static void kinetic_mineral(const pfchem_network* net, orc_cell* c, double* res,
         for (int i = 0; i < ncs; ++i)
           jac[ids[1 + i] * n + jc] +=
               st[i] * dIm_dQK * (1.0 + QK / limiter / den) * dQK_dmj / den;
+      }
+    }
+    if (npref > 0) {
+      /* derivative through the prefactors (:964-1033, primary species) */
+      const double dIm_dsum_prefactor_rate = Im / sum_prefactor_rate;
+      for (int p = 0; p < npref; ++p) {
+        double arrhenius = 1.0;
+        const double Ea = net->kinmnrl_pref_activation_energy
+                              ? net->kinmnrl_pref_activation_energy[(size_t)m * mp + p]
+                              : 0.0;
+        if (Ea > 0.0)
+          arrhenius = exp(Ea / IDEAL_GAS_CONSTANT *
+                          (1.0 / (25.0 + 273.15) - 1.0 / (c->temp + 273.15)));
+        const double ln_prefactor = log(prefactor[p]);
+        const int32_t* pid = net->kinmnrl_prefactor_id + ((size_t)m * mp + p) * (msp + 1);
+        for (int q = 0; q < pid[0]; ++q) {
+          const size_t k = ((size_t)m * mp + p) * msp + q;
+          const int icomp = pid[1 + q];
+          const double ln_spec_act = ln_act[icomp];
+          const double ln_prefactor_spec =
+              net->kinmnrl_pref_alpha[k] * ln_spec_act -
+              log(1.0 + exp(log(net->kinmnrl_pref_atten_coef[k]) +
+                            net->kinmnrl_pref_beta[k] * ln_spec_act));
+          const double dprefactor_dprefactor_spec = exp(ln_prefactor - ln_prefactor_spec);
+          const double dnum = net->kinmnrl_pref_alpha[k] * exp(ln_prefactor_spec - ln_spec_act);
+          const double ln_gam_m_beta = net->kinmnrl_pref_beta[k] * ln_spec_act;
+          const double denominator =
+              1.0 + exp(log(net->kinmnrl_pref_atten_coef[k]) + ln_gam_m_beta);
+          const double dden = -1.0 * exp(ln_prefactor_spec) / denominator *
+                              net->kinmnrl_pref_atten_coef[k] * net->kinmnrl_pref_beta[k] *
+                              exp(ln_gam_m_beta - ln_spec_act);
+          const double dspec = (dnum + dden) * c->pri_act_coef[icomp];
+          const double dIm_dspec = dIm_dsum_prefactor_rate * dprefactor_dprefactor_spec * dspec *
+                                   net->kinmnrl_pref_rate[(size_t)m * mp + p] * arrhenius;
+          for (int i = 0; i < ncs; ++i) jac[ids[1 + i] * n + icomp] += st[i] * dIm_dspec;
+        }
       }
     }
   }

@@ -306,6 +306,40 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
       delete ctx;
       return PFCHEM_ERR_INVALID;
     }
+  if (net->kinmnrl_num_prefactors && net->nkinmnrl > 0) {
+    // rate prefactors: strides, counts and primary-species ids
+    const int mp = net->kinmnrl_max_pref, msp = net->kinmnrl_max_pref_spec;
+    if (mp < 1 || mp > 8 || msp < 1 || msp > 8 || !net->kinmnrl_prefactor_id ||
+        !net->kinmnrl_pref_alpha || !net->kinmnrl_pref_beta || !net->kinmnrl_pref_atten_coef ||
+        !net->kinmnrl_pref_rate) {
+      delete ctx;
+      return PFCHEM_ERR_INVALID;
+    }
+    for (int m = 0; m < net->nkinmnrl; ++m) {
+      const int np = net->kinmnrl_num_prefactors[m];
+      if (np < 0 || np > mp) {
+        delete ctx;
+        return PFCHEM_ERR_INVALID;
+      }
+      for (int p = 0; p < np; ++p) {
+        const int32_t* pid = net->kinmnrl_prefactor_id + ((size_t)m * mp + p) * (msp + 1);
+        if (pid[0] < 0 || pid[0] > msp) {
+          delete ctx;
+          return PFCHEM_ERR_INVALID;
+        }
+        for (int q = 0; q < pid[0]; ++q) {
+          if (pid[1 + q] < 0) {  // a complex (the reference's negative ids)
+            delete ctx;
+            return PFCHEM_ERR_UNSUPPORTED;
+          }
+          if (pid[1 + q] >= naq) {
+            delete ctx;
+            return PFCHEM_ERR_INVALID;
+          }
+        }
+      }
+    }
+  }
   if (net->somdec) {
     const int rc = check_somdec(net, msg);
     if (rc) {
@@ -364,6 +398,17 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   size_t omS = addd(net->kinmnrl_min_scale_factor, nm), omP = addd(net->kinmnrl_affinity_power, nm);
   size_t omt = addd(net->kinmnrl_affinity_threshold, nm), oml = addd(net->kinmnrl_rate_limiter, nm);
   size_t omI = addi(net->kinmnrl_irreversible, nm);
+  // rate prefactors
+  const bool pref = net->kinmnrl_num_prefactors && nm > 0;
+  const size_t pmp = pref ? (size_t)net->kinmnrl_max_pref : 0;
+  const size_t pms = pref ? (size_t)net->kinmnrl_max_pref_spec : 0;
+  size_t opn = pref ? addi(net->kinmnrl_num_prefactors, nm) : kNone;
+  size_t opi = pref ? addi(net->kinmnrl_prefactor_id, nm * pmp * (pms + 1)) : kNone;
+  size_t opa = pref ? addd(net->kinmnrl_pref_alpha, nm * pmp * pms) : kNone;
+  size_t opb = pref ? addd(net->kinmnrl_pref_beta, nm * pmp * pms) : kNone;
+  size_t opt_ = pref ? addd(net->kinmnrl_pref_atten_coef, nm * pmp * pms) : kNone;
+  size_t opr = pref ? addd(net->kinmnrl_pref_rate, nm * pmp) : kNone;
+  size_t ope = pref ? addd(net->kinmnrl_pref_activation_energy, nm * pmp) : kNone;
   size_t odi = addi(net->decayspecid, net->ndecay_rxn), odk = addd(net->decay_rate_constant, net->ndecay_rxn);
   // largest species id per complex / mineral (per-cell active primaries)
   size_t ocmx = kNone, ommx = kNone;
@@ -534,6 +579,16 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   d.mnrl_irrev = pi(omI);
   d.decay_id = pi(odi);
   d.decay_k = pd(odk);
+  d.has_pref = pref ? 1 : 0;
+  d.pref_mp = (int)pmp;
+  d.pref_msp = (int)pms;
+  d.pref_n = pi(opn);
+  d.pref_id = pi(opi);
+  d.pref_alpha = pd(opa);
+  d.pref_beta = pd(opb);
+  d.pref_atten = pd(opt_);
+  d.pref_rate = pd(opr);
+  d.pref_Ea = pd(ope);
   d.variable = opt->variable_ncomp != 0;
   d.cplx_maxid = pi(ocmx);
   d.mnrl_maxid = pi(ommx);

@@ -116,6 +116,16 @@ struct DevNet {
   int variable;
   const int* cplx_maxid;
   const int* mnrl_maxid;
+  // kinetic-mineral rate prefactors (kinmnrl_num_prefactors ...): strides
+  // mp (prefactors per mineral) and msp (species per prefactor)
+  int has_pref, pref_mp, pref_msp;
+  const int* pref_n;         // [nm]
+  const int* pref_id;        // [nm][mp][msp + 1], [..][0] = count
+  const double* pref_alpha;  // [nm][mp][msp]
+  const double* pref_beta;
+  const double* pref_atten;
+  const double* pref_rate;   // [nm][mp]
+  const double* pref_Ea;     // [nm][mp] or null
 };
 
 // SoA per-cell state, device pointers (pfchem_state)
@@ -172,6 +182,54 @@ __device__ __forceinline__ double interp_logk(const double* b, double temp) {
   double tk = temp + 273.15;
   return gld(b) * log(tk) + gld(b + 1) + gld(b + 2) * tk + gld(b + 3) / tk +
          gld(b + 4) / (tk * tk);
+}
+
+// Rate prefactors of mineral m (reaction_mineral.F90:820-860) at ln
+// activities lnact: the sum over its prefactors p of
+// rate_p Arr_p prod_s a_s^alpha / (1 + atten a_s^beta); 0 prefactors: -1.
+__device__ __forceinline__ double pref_arrhenius(const DevNet& n, int m, int p, double temp) {
+  const double Ea = n.pref_Ea ? gld(n.pref_Ea + m * n.pref_mp + p) : 0.0;
+  return Ea > 0.0 ? exp(Ea / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15))) : 1.0;
+}
+__device__ __forceinline__ double pref_ln_term(const DevNet& n, size_t k, double lna) {
+  return gld(n.pref_alpha + k) * lna -
+         log(1.0 + exp(log(gld(n.pref_atten + k)) + gld(n.pref_beta + k) * lna));
+}
+template <typename A>
+__device__ double prefactor_sum(const DevNet& n, int m, const A* lnact, double temp) {
+  const int np = gld(n.pref_n + m);
+  double sum = 0.0;
+  for (int p = 0; p < np; ++p) {
+    const int* pid = n.pref_id + ((size_t)m * n.pref_mp + p) * (n.pref_msp + 1);
+    double lp = 0.0;
+    const int ns = gld(pid);
+    for (int q = 0; q < ns; ++q)
+      lp += pref_ln_term(n, ((size_t)m * n.pref_mp + p) * n.pref_msp + q, lnact[gld(pid + 1 + q)]);
+    sum += exp(lp) * gld(n.pref_rate + m * n.pref_mp + p) * pref_arrhenius(n, m, p, temp);
+  }
+  return sum;
+}
+// Jacobian terms of the prefactors (reaction_mineral.F90:964-1033): for each
+// prefactor species j, d(Im V)/d ln m_j = IV / sum * rate_p Arr_p prefactor_p
+// * (alpha - atten beta a^beta / (1 + atten a^beta)); add(j, v) receives them
+template <typename A, typename F>
+__device__ void prefactor_jacobian(const DevNet& n, int m, const A* lnact, double temp,
+                                   double IV_over_sum, F&& add) {
+  const int np = gld(n.pref_n + m);
+  for (int p = 0; p < np; ++p) {
+    const size_t base = (size_t)m * n.pref_mp + p;
+    const int* pid = n.pref_id + base * (n.pref_msp + 1);
+    const int ns = gld(pid);
+    double lp = 0.0;
+    for (int q = 0; q < ns; ++q) lp += pref_ln_term(n, base * n.pref_msp + q, lnact[gld(pid + 1 + q)]);
+    const double c0 = IV_over_sum * exp(lp) * gld(n.pref_rate + base) * pref_arrhenius(n, m, p, temp);
+    for (int q = 0; q < ns; ++q) {
+      const size_t k = base * n.pref_msp + q;
+      const int j = gld(pid + 1 + q);
+      const double ab = gld(n.pref_atten + k) * exp(gld(n.pref_beta + k) * lnact[j]);
+      add(j, c0 * (gld(n.pref_alpha + k) - ab * gld(n.pref_beta + k) / (1.0 + ab)));
+    }
+  }
 }
 
 // Debye-Hueckel ln(gamma) * 1/ln10 -> gamma (reaction.F90:4586-4592)

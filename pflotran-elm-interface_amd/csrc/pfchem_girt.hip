@@ -209,7 +209,7 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         const int ncs = gld(ids);
         for (int a = 0; a < ncs; ++a) lnQK += gld(st + a) * lnact[gld(ids + 1 + a)];
         const MineralRate mr =
-            mineral_rate(n, m, lnQK, gld(s.mnrl_volfrac + (m * ld + c)), area, temp);
+            mineral_rate(n, m, lnQK, gld(s.mnrl_volfrac + (m * ld + c)), area, temp, lnact);
         if (mr.active) {
           rt = mr.Im;
           const double Iv = mr.Im * vol;
@@ -223,6 +223,12 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
               atomicAdd(&J[ia * (NM + 1) + ib], nua * g * gld(st + b) * xinv[ib]);
             }
           }
+          // rate prefactors (reaction_mineral.F90:964-1033)
+          if (n.has_pref && gld(n.pref_n + m) > 0)
+            prefactor_jacobian(n, m, lnact, temp, Iv / mr.kr, [&](int j, double v) {
+              for (int a = 0; a < ncs; ++a)
+                atomicAdd(&J[gld(ids + 1 + a) * (NM + 1) + j], gld(st + a) * v * xinv[j]);
+            });
         }
       }
       if (s.mnrl_rate) gst(s.mnrl_rate + (m * ld + c), rt);

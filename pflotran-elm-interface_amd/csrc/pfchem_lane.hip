@@ -147,6 +147,13 @@ int lane_supported(const pfchem_network* net, const pfchem_options* opt, std::st
   }
   if (net->neqcplx > 0 && net->eqcplx_maxsp > 8) { why = "complex rows longer than 8"; return 0; }
   if (net->nkinmnrl > 0 && net->kinmnrl_maxsp > 8) { why = "mineral rows longer than 8"; return 0; }
+  if (net->kinmnrl_num_prefactors)
+    for (int m = 0; m < net->nkinmnrl; ++m)
+      if (net->kinmnrl_num_prefactors[m] > 0) {
+        // their Jacobian terms are not symmetric
+        why = "mineral rate prefactors";
+        return 0;
+      }
   if (opt->act_coef_update_algorithm == PFCHEM_ACT_COEF_ALGORITHM_NEWTON &&
       opt->act_coef_update_frequency == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER) {
     why = "NEWTON activity algorithm inside the Newton iteration";

@@ -126,10 +126,16 @@ __device__ void plain_minerals(const DevNet& n, const DevState& s, int64_t c,
       continue;
     const double lim = n.has_limiter ? n.mnrl_limiter[m] : 0.0;
     if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
-    double arr = 1.0;
-    if (n.has_Ea && n.mnrl_Ea[m] > 0.0)
-      arr = exp(n.mnrl_Ea[m] / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (w.temp + 273.15)));
-    const double kr = n.mnrl_k[m] * arr;
+    const bool pref = n.has_pref && n.pref_n[m] > 0;
+    double kr;
+    if (pref) {
+      kr = prefactor_sum(n, m, lna, w.temp);  // reaction_mineral.F90:820-860
+    } else {
+      double arr = 1.0;
+      if (n.has_Ea && n.mnrl_Ea[m] > 0.0)
+        arr = exp(n.mnrl_Ea[m] / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (w.temp + 273.15)));
+      kr = n.mnrl_k[m] * arr;
+    }
     double Im_const = -s.mnrl_area[m * s.ld + c];
     if (n.has_minscale) Im_const = Im_const / n.mnrl_minscale[m];
     double Im = n.has_affpower
@@ -162,6 +168,11 @@ __device__ void plain_minerals(const DevNet& n, const DevState& s, int64_t c,
           J[ids[1 + i] * N + jc] += st[i] * dIm_dQK * dQK_dmj;
       }
     }
+    // rate prefactors (reaction_mineral.F90:964-1033), d/dm = d/dln m / m
+    if (pref)
+      prefactor_jacobian(n, m, lna, w.temp, Im / kr, [&](int j, double v) {
+        for (int i = 0; i < ncs; ++i) J[ids[1 + i] * N + j] += st[i] * v / w.x[j];
+      });
   }
 }
 

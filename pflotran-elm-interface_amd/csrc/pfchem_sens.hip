@@ -527,7 +527,7 @@ int sens_supported(const DevNet& n, const pfchem_options& opt) {
   if (n.variable || n.nimm > 0 || n.nm > kSensMaxMnrl || n.ncomp > 32) return 0;
   if (opt.act_coef_update_frequency == PFCHEM_ACT_COEF_FREQUENCY_NEWTON_ITER) return 0;
   if (n.has_temkin || n.has_minscale || n.has_affpower || n.has_limiter || n.has_threshold ||
-      n.has_irrev)
+      n.has_irrev || n.has_pref)
     return 0;
   return 1;
 }

@@ -206,6 +206,7 @@ __host__ __device__ inline TeamLds team_lds_layout(int NM, int nc, int nm, int j
 // and dIm/dQK per unit volume, with the affinity options.
 struct MineralRate {
   double Im, dIm_dQK, QK;
+  double kr;  // rate constant with its Arrhenius factor, or the prefactor sum
   bool active;
 };
 
@@ -237,12 +238,13 @@ __device__ __forceinline__ void rate_options(const DevNet& n, int m, double QK, 
   r->dIm_dQK = d;
 }
 
+template <typename A = double>
 __device__ __forceinline__ MineralRate mineral_rate(const DevNet& n, int m, double lnQK,
                                                     double volfrac, double area,
-                                                    double temp) {
+                                                    double temp, const A* lnact = nullptr) {
   MineralRate r;
   r.active = false;
-  r.Im = r.dIm_dQK = 0.0;
+  r.Im = r.dIm_dQK = r.kr = 0.0;
   const double QK = exp(lnQK);
   r.QK = QK;
   const bool plain_aff = !(n.has_temkin || n.has_minscale);
@@ -256,13 +258,19 @@ __device__ __forceinline__ MineralRate mineral_rate(const DevNet& n, int m, doub
   }
   const double lim = n.has_limiter ? gld(n.mnrl_limiter + m) : 0.0;
   if (lim > 0.0) aff = aff / (1.0 + (1.0 - aff) / lim);
-  double arr = 1.0;
-  if (n.has_Ea) {
-    const double Ea = gld(n.mnrl_Ea + m);
-    if (Ea > 0.0)
-      arr = exp(Ea / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
+  double kr;
+  if (n.has_pref && lnact && gld(n.pref_n + m) > 0) {
+    kr = prefactor_sum(n, m, lnact, temp);  // reaction_mineral.F90:820-860
+  } else {
+    double arr = 1.0;
+    if (n.has_Ea) {
+      const double Ea = gld(n.mnrl_Ea + m);
+      if (Ea > 0.0)
+        arr = exp(Ea / kIdealGas * (1.0 / (25.0 + 273.15) - 1.0 / (temp + 273.15)));
+    }
+    kr = gld(n.mnrl_k + m) * arr;
   }
-  const double kr = gld(n.mnrl_k + m) * arr;
+  r.kr = kr;
   double Im_const = -area;
   if (n.has_minscale) Im_const = Im_const / gld(n.mnrl_minscale + m);
   r.active = true;
@@ -356,6 +364,22 @@ struct RxnRow {
     }
   }
 };
+
+// prefactor Jacobian terms of mineral m into the team's tile (scaled log
+// form: d res_i / d ln m_j times the row scale of i)
+template <int NM>
+__device__ void team_prefactor_jacobian(const DevNet& n, int m, const double* lnact, double temp,
+                                        double IV_over_sum, double* Jt, const double* rsv) {
+  const int* ids = n.mnrl_ids + m * (n.mnrl_ms + 1);
+  const double* st = n.mnrl_st + m * n.mnrl_ms;
+  const int ncs = gld(ids);
+  prefactor_jacobian(n, m, lnact, temp, IV_over_sum, [&](int j, double v) {
+    for (int a = 0; a < ncs; ++a) {
+      const int i = gld(ids + 1 + a);
+      atomicAdd(&Jt[i * (NM + 1) + j], gld(st + a) * v * rsv[i]);
+    }
+  });
+}
 
 // SomDec accumulator of a team (see pfchem_somdec.hpp): species values from
 // the team's LDS tile, residual terms into the reaction residual, Jacobian
@@ -802,12 +826,12 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
     const int cms = n.cplx_ms, mms = n.mnrl_ms;
     // minerals m = q*L + t (q < KM): rates and scatter weights (reads only)
     RxnRow<kRowSlots> mrow[KM];
-    double mv[KM], mg[KM];
+    double mv[KM], mg[KM], mkr[KM];
     bool mon[KM];
 #pragma unroll
     for (int q = 0; q < KM; ++q) {
       const int m = q * L + t;
-      mv[q] = mg[q] = 0.0;
+      mv[q] = mg[q] = mkr[q] = 0.0;
       mon[q] = false;
       if (m < nm) {
         const double logK = n.isothermal || !n.has_mnrl_logKcoef
@@ -819,13 +843,14 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
             mrow[q].load(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, lnact, rsv, lnQK0);
         double rt = 0.0;
         if (area[m] != 0.0) {  // zero-area minerals give Im = 0
-          const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp);
+          const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp, lnact);
           if (mr.active) {
             rt = mr.Im;
             // dry cells: RReaction adds nothing (reaction.F90:4083), but the
             // kinetic-state update still uses this rate
             mon[q] = wet;
             mv[q] = mr.Im * vol;
+            mkr[q] = mr.kr;
             // log form: d res_i / d ln m_j = nu_i dIm/dQK nu_j QK * rho/1000
             mg[q] = mr.dIm_dQK * vol * mr.QK * dkl;
           }
@@ -865,9 +890,12 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 #pragma unroll
     for (int q = 0; q < KM; ++q) {
       const int m = q * L + t;
-      if (m < nm && mon[q])
+      if (m < nm && mon[q]) {
         mrow[q].template scatter<NM>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, mv[q],
                                          mg[q], true, racc, Jt, rsv);
+        if (n.has_pref && gld(n.pref_n + m) > 0)
+          team_prefactor_jacobian<NM>(n, m, lnact, temp, mv[q] / mkr[q], Jt, rsv);
+      }
     }
     // minerals beyond the register slots (more than KM*L of them)
     for (int m = KM * L + t; m < nm; m += L) {
@@ -880,13 +908,16 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         double lnQK = -logK * kLogToLn;
         if (n.has_mnrl_h2o) lnQK += gld(n.mnrl_h2o + m) * ln_aw;
         lnQK = row.load(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms, lnact, rsv, lnQK);
-        const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp);
+        const MineralRate mr = mineral_rate(n, m, lnQK, vf[m], area[m], temp, lnact);
         if (mr.active) {
           rt = mr.Im;
-          if (wet)
+          if (wet) {
             row.template scatter<NM>(n.mnrl_ids + m * (mms + 1), n.mnrl_st + m * mms,
                                          mr.Im * vol, mr.dIm_dQK * vol * mr.QK * dkl, true, racc,
                                          Jt, rsv);
+            if (n.has_pref && gld(n.pref_n + m) > 0)
+              team_prefactor_jacobian<NM>(n, m, lnact, temp, mr.Im * vol / mr.kr, Jt, rsv);
+          }
         }
       }
       rate[m] = rt;

@@ -92,6 +92,11 @@ module pfchem_c
     real(c_double) :: debyeB
     real(c_double) :: debyeBdot
     type(c_ptr) :: somdec
+    ! kinetic-mineral rate prefactors (mineral%kinmnrl_num_prefactors ...)
+    integer(c_int32_t) :: kinmnrl_max_pref, kinmnrl_max_pref_spec
+    type(c_ptr) :: kinmnrl_num_prefactors, kinmnrl_prefactor_id
+    type(c_ptr) :: kinmnrl_pref_alpha, kinmnrl_pref_beta, kinmnrl_pref_atten_coef
+    type(c_ptr) :: kinmnrl_pref_rate, kinmnrl_pref_activation_energy
   end type pfchem_network
 
   type, bind(C) :: pfchem_options

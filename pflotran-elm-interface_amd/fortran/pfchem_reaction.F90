@@ -52,6 +52,14 @@ module pfchem_reaction_module
     real(dp), pointer :: kinmnrl_affinity_threshold(:) => null()
     real(dp), pointer :: kinmnrl_rate_limiter(:) => null()
     integer, pointer :: kinmnrl_irreversible(:) => null()
+    ! rate prefactors (PREFACTOR blocks)
+    integer, pointer :: kinmnrl_num_prefactors(:) => null()          ! (nkinmnrl)
+    integer, pointer :: kinmnrl_prefactor_id(:,:,:) => null()        ! (0:maxspec, maxpref, nkinmnrl)
+    real(dp), pointer :: kinmnrl_pref_alpha(:,:,:) => null()         ! (maxspec, maxpref, nkinmnrl)
+    real(dp), pointer :: kinmnrl_pref_beta(:,:,:) => null()
+    real(dp), pointer :: kinmnrl_pref_atten_coef(:,:,:) => null()
+    real(dp), pointer :: kinmnrl_pref_rate(:,:) => null()            ! (maxpref, nkinmnrl)
+    real(dp), pointer :: kinmnrl_pref_activation_energy(:,:) => null()
   end type mineral_type
 
   type, public :: immobile_type
@@ -190,8 +198,9 @@ contains
     type(pfchem_network) :: net
     type(pfchem_options) :: opt
     integer(c_int32_t), allocatable, target, save :: cspec(:,:), mspec(:,:), dspec(:), irrev(:)
+    integer(c_int32_t), allocatable, target, save :: pnum(:), pid(:,:,:)
     real(dp), allocatable, target, save :: ch2o(:), mh2o(:)
-    integer :: k, ms, nimm
+    integer :: k, ms, nimm, mp, msp
     integer(c_int) :: rc
 
     ierror = 0
@@ -261,6 +270,17 @@ contains
     net%kinmnrl_affinity_threshold = c_null_ptr
     net%kinmnrl_rate_limiter = c_null_ptr
     net%kinmnrl_irreversible = c_null_ptr
+    net%kinmnrl_max_pref = 0
+    net%kinmnrl_max_pref_spec = 0
+    net%kinmnrl_num_prefactors = c_null_ptr
+    net%kinmnrl_prefactor_id = c_null_ptr
+    net%kinmnrl_pref_alpha = c_null_ptr
+    net%kinmnrl_pref_beta = c_null_ptr
+    net%kinmnrl_pref_atten_coef = c_null_ptr
+    net%kinmnrl_pref_rate = c_null_ptr
+    net%kinmnrl_pref_activation_energy = c_null_ptr
+    if (allocated(pnum)) deallocate(pnum)
+    if (allocated(pid)) deallocate(pid)
     if (allocated(mspec)) deallocate(mspec)
     if (allocated(mh2o)) deallocate(mh2o)
     if (allocated(irrev)) deallocate(irrev)
@@ -302,6 +322,29 @@ contains
             allocate(irrev(m%nkinmnrl))
             irrev = int(m%kinmnrl_irreversible(1:m%nkinmnrl), c_int32_t)
             net%kinmnrl_irreversible = c_loc(irrev)
+          end if
+          ! rate prefactors: same memory order; primary ids -> 0-based
+          ! (negative ids name complexes and stay negative)
+          if (associated(m%kinmnrl_num_prefactors)) then
+            if (maxval(m%kinmnrl_num_prefactors(1:m%nkinmnrl)) > 0) then
+              msp = size(m%kinmnrl_prefactor_id, 1) - 1
+              mp = size(m%kinmnrl_prefactor_id, 2)
+              allocate(pnum(m%nkinmnrl))
+              pnum = int(m%kinmnrl_num_prefactors(1:m%nkinmnrl), c_int32_t)
+              allocate(pid(0:msp, mp, m%nkinmnrl))
+              pid = int(m%kinmnrl_prefactor_id(:, :, 1:m%nkinmnrl), c_int32_t)
+              where (pid(1:msp, :, :) > 0) pid(1:msp, :, :) = pid(1:msp, :, :) - 1
+              net%kinmnrl_max_pref = mp
+              net%kinmnrl_max_pref_spec = msp
+              net%kinmnrl_num_prefactors = c_loc(pnum)
+              net%kinmnrl_prefactor_id = c_loc(pid)
+              net%kinmnrl_pref_alpha = c_loc(m%kinmnrl_pref_alpha)
+              net%kinmnrl_pref_beta = c_loc(m%kinmnrl_pref_beta)
+              net%kinmnrl_pref_atten_coef = c_loc(m%kinmnrl_pref_atten_coef)
+              net%kinmnrl_pref_rate = c_loc(m%kinmnrl_pref_rate)
+              if (associated(m%kinmnrl_pref_activation_energy)) &
+                net%kinmnrl_pref_activation_energy = c_loc(m%kinmnrl_pref_activation_energy)
+            end if
           end if
         end if
       end associate

@@ -187,7 +187,8 @@ contains
     write(*, '(*(g0,:," "))') 'network', c_sizeof(net), &
       transfer(c_loc(net%neqcplx), b) - b, transfer(c_loc(net%nkinmnrl), b) - b, &
       transfer(c_loc(net%ndecay_rxn), b) - b, transfer(c_loc(net%debyeA), b) - b, &
-      transfer(c_loc(net%somdec), b) - b
+      transfer(c_loc(net%somdec), b) - b, transfer(c_loc(net%kinmnrl_max_pref), b) - b, &
+      transfer(c_loc(net%kinmnrl_pref_activation_energy), b) - b
     b = transfer(c_loc(opt), b)
     write(*, '(*(g0,:," "))') 'options', c_sizeof(opt), &
       transfer(c_loc(opt%max_dlnC_rreact), b) - b, &

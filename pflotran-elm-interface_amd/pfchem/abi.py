@@ -160,6 +160,15 @@ class Network(C.Structure):
         ("debyeB", C.c_double),
         ("debyeBdot", C.c_double),
         ("somdec", C.POINTER(Somdec)),
+        ("kinmnrl_max_pref", C.c_int32),
+        ("kinmnrl_max_pref_spec", C.c_int32),
+        ("kinmnrl_num_prefactors", _ip),
+        ("kinmnrl_prefactor_id", _ip),
+        ("kinmnrl_pref_alpha", _dp),
+        ("kinmnrl_pref_beta", _dp),
+        ("kinmnrl_pref_atten_coef", _dp),
+        ("kinmnrl_pref_rate", _dp),
+        ("kinmnrl_pref_activation_energy", _dp),
     ]
 
 

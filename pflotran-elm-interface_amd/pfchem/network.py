@@ -94,6 +94,8 @@ class ReactionNetwork:
     debyeB: float = DEBYE_B
     debyeBdot: float = DEBYE_BDOT
     somdec: Optional[object] = None  # pfchem.somdec.SomDecNetwork
+    # rate prefactors per kinetic mineral (None or [] = the rate constant)
+    kinmnrl_prefactors: Optional[List[List["Prefactor"]]] = None
 
     def __post_init__(self):
         self.primary_spec_Z = _f64(self.primary_spec_Z)
@@ -241,6 +243,39 @@ class ReactionNetwork:
         s.debyeA = self.debyeA
         s.debyeB = self.debyeB
         s.debyeBdot = self.debyeBdot
+        pref = self.kinmnrl_prefactors
+        if pref and any(pref):
+            nm = self.nkinmnrl
+            mp = max(len(p) for p in pref)
+            msp = max(max((len(q.species) for q in p), default=0) for p in pref)
+            msp = max(msp, 1)
+            npf = np.zeros(nm, np.int32)
+            pid = np.zeros((nm, mp, msp + 1), np.int32)
+            alpha = np.zeros((nm, mp, msp))
+            beta = np.zeros((nm, mp, msp))
+            atten = np.zeros((nm, mp, msp))
+            prate = np.zeros((nm, mp))
+            pea = np.zeros((nm, mp))
+            for m, plist in enumerate(pref):
+                npf[m] = len(plist)
+                for q, pf in enumerate(plist):
+                    prate[m, q] = pf.rate
+                    pea[m, q] = pf.activation_energy
+                    pid[m, q, 0] = len(pf.species)
+                    for a_, (sid, al, be, at) in enumerate(pf.species):
+                        pid[m, q, 1 + a_] = sid
+                        alpha[m, q, a_] = al
+                        beta[m, q, a_] = be
+                        atten[m, q, a_] = at
+            s.kinmnrl_max_pref = mp
+            s.kinmnrl_max_pref_spec = msp
+            s.kinmnrl_num_prefactors = ip(npf)
+            s.kinmnrl_prefactor_id = ip(pid)
+            s.kinmnrl_pref_alpha = dp(alpha)
+            s.kinmnrl_pref_beta = dp(beta)
+            s.kinmnrl_pref_atten_coef = dp(atten)
+            s.kinmnrl_pref_rate = dp(prate)
+            s.kinmnrl_pref_activation_energy = dp(pea)
         if self.somdec is not None:
             sd = self.somdec.to_abi(keep)
             keep.append(sd)
@@ -250,6 +285,17 @@ class ReactionNetwork:
         keep.append(s)
         self._abi_keep = keep
         return s
+
+
+@dataclass
+class Prefactor:
+    """one parallel rate term of a kinetic mineral (a PREFACTOR block of
+    MINERAL_KINETICS, reaction_mineral.F90:820-870): rate constant (mol/m^2/s),
+    activation energy (J/mol, 0 = none) and species terms (primary id, ALPHA,
+    BETA, ATTENUATION_COEF): a^alpha / (1 + atten a^beta)"""
+    rate: float
+    species: List[tuple] = field(default_factory=list)
+    activation_energy: float = 0.0
 
 
 @dataclass

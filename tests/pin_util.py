@@ -66,3 +66,21 @@ def mineral_options_workload(ncell=512, **opts):
     arrays = {k: (np.array([v], dtype=np.int32) if k == "kinmnrl_irreversible"
                   else np.array([float(v)])) for k, v in opts.items()}
     return dataclasses.replace(wl, net=dataclasses.replace(wl.net, **arrays))
+
+
+def prefactor_workload(ncell=512, seed=5):
+    """the calcite workload with calcite's rate given by two parallel rate
+    prefactors (PREFACTOR blocks, reaction_mineral.F90:820-870):
+    1e-5 a(H+)^0.37 / (1 + 1e9 a(H+)) a(Ca++)^0.31 and, with an activation
+    energy of 20 kJ/mol, 3e-8 a(HCO3-)^0.5; per-cell temperatures U[15, 45] C
+    (log K stays at its 25 C value: isothermal options)"""
+    from pfchem.network import Prefactor
+    wl = workloads.calcite_workload(ncell)
+    net = wl.net
+    ids = {n: i for i, n in enumerate(net.primary_species_names)}
+    pf = [[Prefactor(1.0e-5, [(ids["H+"], 0.37, 1.0, 1.0e9), (ids["Ca++"], 0.31, 0.0, 0.0)]),
+           Prefactor(3.0e-8, [(ids["HCO3-"], 0.5, 0.0, 0.0)], activation_energy=2.0e4)]]
+    st = wl.state.copy()
+    rng = np.random.default_rng(seed)
+    st.arrays["temperature"][0, :ncell] = rng.uniform(15.0, 45.0, ncell)
+    return dataclasses.replace(wl, net=dataclasses.replace(net, kinmnrl_prefactors=pf), state=st)

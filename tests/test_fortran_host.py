@@ -28,10 +28,11 @@ C_LAYOUT = r"""
 #include <stdio.h>
 #include "pfchem.h"
 int main(void) {
-  printf("network %zu %zu %zu %zu %zu %zu\n", sizeof(pfchem_network),
+  printf("network %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(pfchem_network),
          offsetof(pfchem_network, neqcplx), offsetof(pfchem_network, nkinmnrl),
          offsetof(pfchem_network, ndecay_rxn), offsetof(pfchem_network, debyeA),
-         offsetof(pfchem_network, somdec));
+         offsetof(pfchem_network, somdec), offsetof(pfchem_network, kinmnrl_max_pref),
+         offsetof(pfchem_network, kinmnrl_pref_activation_energy));
   printf("options %zu %zu %zu\n", sizeof(pfchem_options),
          offsetof(pfchem_options, max_dlnC_rreact),
          offsetof(pfchem_options, max_relative_change_tolerance));

@@ -2,7 +2,9 @@
 immobile species with first-order decay (RImmobileDecay,
 reaction_immobile.F90:244-294), temperature-dependent log K with an Arrhenius
 rate constant (ReactionInterpolateLogK, reaction_aux.F90:1285-1312), the
-kinetic-mineral rate options (reaction_mineral.F90:729-908), and
+kinetic-mineral rate options (reaction_mineral.F90:729-908) and rate
+prefactors (:820-870, 964-1033; the one-lane kernel does not take them, so
+the batched variants run the team kernel), and
 RActivityCoefficients LAG through pfchem_update_activity
 (reaction.F90:4555-4611).  Every kernel variant and both batched kernels (the
 one-lane kernel, the default, and a 16x1 team) run through the C ABI against
@@ -60,6 +62,12 @@ def test_decay(variant, lanes):
 def test_nonisothermal(variant, lanes):
     wl = PU.nonisothermal_workload(512)
     _parity(wl, variant, lanes, "nonisothermal/" + IDS[KERNELS.index((variant, lanes))])
+
+
+@pytest.mark.parametrize("variant,lanes", KERNELS, ids=IDS)
+def test_prefactors(variant, lanes):
+    wl = PU.prefactor_workload(512)
+    _parity(wl, variant, lanes, "prefactor/" + IDS[KERNELS.index((variant, lanes))])
 
 
 OPTION_SETS = [dict(kinmnrl_Temkin_const=2.0, kinmnrl_min_scale_factor=1.5),

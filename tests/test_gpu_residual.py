@@ -32,6 +32,9 @@ def _case(which, ncell, seed=7):
         wl = workloads.calcite_workload(ncell)
     elif which == "somdec":
         wl = workloads.somdec_workload(ncell=ncell)
+    elif which == "prefactor":
+        import pin_util as PU
+        wl = PU.prefactor_workload(ncell)
     else:
         wl = workloads.random_network_workload(ncell=ncell)
     st = wl.state.copy()
@@ -80,7 +83,8 @@ def _oracle(wl, st, fixed, fimm, cells, dt):
     return res, jac, tot, sec, acc
 
 
-@pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048), ("somdec", 1024)])
+@pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048), ("somdec", 1024),
+                                         ("prefactor", 512)])
 def test_residual_jacobian_matches_oracle(which, ncell):
     import torch
     from pfchem.lib import Context, DeviceState

@@ -93,10 +93,14 @@ def test_hand_over_without_later_pass(calcite_host):
 
 def test_prepare_refuses_other_networks():
     """the one-lane kernel takes aqueous networks of <= 16 primaries without
-    the sandbox or per-cell active counts; the others report UNSUPPORTED"""
+    the sandbox, per-cell active counts or rate prefactors (whose Jacobian
+    terms are not symmetric); the others report UNSUPPORTED"""
     wl = workloads.variable_network_workload(ncell=8)
     assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
     wl = workloads.somdec_workload(ncell=8)
+    assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
+    import pin_util as PU
+    wl = PU.prefactor_workload(8)
     assert lib.prepare_kernels(wl.net, wl.opt) == abi.PFCHEM_ERR_UNSUPPORTED
 
 

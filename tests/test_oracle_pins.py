@@ -294,3 +294,87 @@ def test_option_jacobian(opts, Q):
     want = np.outer(nu_id, nu_id * Q / m) * dIm * rho
     np.testing.assert_allclose(jac[:3, :3], want, rtol=1e-11, atol=1e-11 * scale)
     assert np.abs(jac[:3, :3] - jn[:3, :3] * rho).max() > 1e-3 * scale
+
+
+# ---------------------------------------------------------------------------
+# rate prefactors (reaction_mineral.F90:820-870, derivative :964-1033)
+import pin_util as PU
+from pfchem.network import Prefactor
+
+
+def test_prefactor_without_species_terms_is_the_rate_constant():
+    """one prefactor with alpha = beta = 0 and no attenuation is exactly the
+    rate constant: the whole step equals the plain TST one"""
+    wl = workloads.calcite_workload(256)
+    net = wl.net
+    k = float(net.kinmnrl_rate_constant[0])
+    net_p = dataclasses.replace(net, kinmnrl_prefactors=[[Prefactor(k, [(0, 0.0, 0.0, 0.0)])]],
+                                kinmnrl_rate_constant=np.array([123.0]))  # unused with prefactors
+    a, b = wl.state.copy(), wl.state.copy()
+    assert O.step_batch(net, wl.opt, a, wl.dt, 0) == 0
+    assert O.step_batch(net_p, wl.opt, b, wl.dt, 0) == 0
+    for f in ("pri_molal", "sec_molal", "mnrl_volfrac", "mnrl_rate"):
+        np.testing.assert_allclose(b[f], a[f], rtol=1e-14, atol=0)
+
+
+def test_prefactor_rate_closed_form():
+    """Im = -A (1 - QK) sum_p k_p Arr_p prod_s a_s^alpha / (1 + atten a_s^beta),
+    Arr = exp(Ea/R (1/298.15 - 1/T)), evaluated in numpy at each cell's state"""
+    wl = PU.prefactor_workload(64)
+    net = wl.net
+    pf = net.kinmnrl_prefactors[0]
+    nu = net.dense_mineral_stoichiometry()[0]
+    checked = 0
+    for cell in range(64):
+        cc = O.CellCtx(net, wl.opt).gather(wl.state, cell)
+        cc.total()
+        cc.reaction(3600.0)
+        rate = cc.get("mnrl_rate", 1)[0]
+        a = cc.get("pri_molal", 3) * cc.get("pri_act_coef", 3)
+        lnQK = -net.kinmnrl_logK[0] * 2.30258509299 + float(nu @ np.log(a))
+        QK = math.exp(lnQK)
+        if not (cc.get("mnrl_volfrac", 1)[0] > 0 or QK > 1):
+            continue
+        T = cc.c.temp + 273.15
+        kr = 0.0
+        for p in pf:
+            f = 1.0
+            for sid, al, be, at in p.species:
+                f *= a[sid] ** al / (1.0 + at * a[sid] ** be)
+            arr = math.exp(p.activation_energy / 8.31446 * (1 / 298.15 - 1 / T))
+            kr += p.rate * arr * f
+        want = -cc.get("mnrl_area", 1)[0] * (1.0 - QK) * kr
+        assert rate == pytest.approx(want, rel=1e-12), cell
+        checked += 1
+    assert checked > 32
+
+
+@pytest.mark.parametrize("alpha_slip", [1.0, 1.1], ids=["as-is", "planted-alpha-slip"])
+def test_prefactor_jacobian_vs_numerical_derivative(alpha_slip):
+    """analytic mineral Jacobian with prefactor terms against RReactionDerivative's
+    forward differences (reaction.F90:41), at unit density so the rho/1000 of
+    the QK block is 1; a 10% slip in one ALPHA in the analytic evaluation only
+    must be seen"""
+    wl = PU.prefactor_workload(16)
+    net = wl.net
+    pf = net.kinmnrl_prefactors[0]
+    slipped = [Prefactor(pf[0].rate, [(pf[0].species[0][0], pf[0].species[0][1] * alpha_slip,
+                                       pf[0].species[0][2], pf[0].species[0][3])] +
+                         pf[0].species[1:]), pf[1]]
+    net_a = dataclasses.replace(net, kinmnrl_prefactors=[slipped])
+    worst = 0.0
+    for cell in range(16):
+        cc = O.CellCtx(net, wl.opt).gather(wl.state, cell)
+        cc.c.den_kg = 1000.0
+        cc.total()
+        _, Jn = cc.reaction_numerical(3600.0)
+        ca = O.CellCtx(net_a, wl.opt).gather(wl.state, cell)
+        ca.c.den_kg = 1000.0
+        ca.total()
+        _, Ja = ca.reaction(3600.0)
+        scale = max(np.abs(Jn).max(), 1e-300)
+        worst = max(worst, float(np.abs(Ja - Jn).max() / scale))
+    if alpha_slip == 1.0:
+        assert worst < 2e-4, worst
+    else:
+        assert worst > 1e-2, worst
