@@ -558,6 +558,57 @@ int pfchem_equilibrate(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state
 int pfchem_equilibrate_static(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                               const pfchem_constraint* con, const double* conc, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* OSRT transport step (the operator-split step before the reaction    */
+/* step): RTCalculateTransportMatrix (reactive_transport.F90:1469-1703),*/
+/* the boundary part of RTCalculateRHS_t1 (:1239-1465) and one linear  */
+/* solve per aqueous component (pmc_subsurface_osrt.F90:286-333).      */
+/* ------------------------------------------------------------------ */
+/* Connection topology (host memory, read by pfchem_transport_setup):
+   internal connections id_up -> id_dn (0-based cells; positive flux from
+   up to dn) and boundary connections into cell bc_cell.  Cells without
+   connections only accumulate. */
+typedef struct pfchem_connections {
+  int64_t nconn;
+  const int64_t* id_up;      /* [nconn] */
+  const int64_t* id_dn;      /* [nconn] */
+  int64_t nbc;
+  const int64_t* bc_cell;    /* [nbc]   */
+} pfchem_connections;
+/* Builds the sparse pattern of the transport matrix on the device (replaces
+   a previous one).  ncell: cells of the states later passed. */
+int pfchem_transport_setup(pfchem_ctx* ctx, int64_t ncell, const pfchem_connections* conn);
+
+/* Per-step coefficients (device memory): the reference's connection area,
+   Darcy flux (patch%internal_velocities / boundary_velocities, m/s) and
+   transport coefficient over distance (patch%internal_tran_coefs /
+   boundary_tran_coefs from RTUpdateTransportCoefs, m/s; the same for every
+   component: species-dependent diffusion is refused by the reference's OSRT
+   too), and the boundary totals (mol/L, [i * nbc + b]). */
+typedef struct pfchem_transport_coefs {
+  const double* area;               /* [nconn] m^2                       */
+  const double* velocity;           /* [nconn] m/s, + from up to dn      */
+  const double* coef_over_dist;     /* [nconn] m/s                       */
+  const double* bc_area;            /* [nbc]                             */
+  const double* bc_velocity;        /* [nbc] m/s, + into the cell        */
+  const double* bc_coef_over_dist;  /* [nbc]                             */
+  const double* bc_total;           /* [naqcomp * nbc]                   */
+} pfchem_transport_coefs;
+/* One implicit transport step of the state's aqueous totals (in place):
+   per component  M c^{k+1} = psv c^k / dt + sum_bc T_up c_bc  with
+   M = psv / dt + the TFluxCoef upwind advection / diffusion coefficients
+   (transport.F90:495-565; psv = porosity sat 1000 volume, totals in mol/L);
+   dry cells (saturation < 1e-40) have no fluxes and keep their totals (an
+   empty row in the reference's matrix).  Solved on the device by
+   BiCGSTAB with a Jacobi preconditioner (all components together) to a
+   relative residual rtol (<= 0: 1e-12) within max_iterations (<= 0: 1000);
+   iterations (host, optional) receives the iterations taken.  Source /
+   sink terms are not supported.  Synchronizes `stream`;
+   PFCHEM_ERR_CELLS: a component did not converge. */
+int pfchem_transport_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
+                          const pfchem_transport_coefs* coefs, double dt, double rtol,
+                          int max_iterations, int32_t* iterations, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

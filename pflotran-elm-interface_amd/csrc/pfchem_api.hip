@@ -13,6 +13,7 @@
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
 #include "pfchem_lane.h"
+#include "pfchem_transport.h"
 #include "pfchem_tune.h"
 
 using namespace pfchem;
@@ -69,6 +70,8 @@ struct pfchem_ctx {
   // network arrays and options
   std::string arch = "gfx";
   uint64_t net_hash = 0;
+  // OSRT transport: connection topology (pfchem_transport_setup)
+  TransportPlan* tplan = nullptr;
   std::string err;
 };
 
@@ -716,6 +719,11 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
 
 namespace pfchem {
 
+int ctx_device(const pfchem_ctx* ctx) { return ctx->device; }
+int ctx_naq(const pfchem_ctx* ctx) { return ctx->net.naqcomp; }
+TransportPlan** ctx_transport_plan(pfchem_ctx* ctx) { return &ctx->tplan; }
+int ctx_fail(pfchem_ctx* ctx, int code, const char* msg) { return fail(ctx, code, msg); }
+
 TuneInfo tune_info(const pfchem_ctx* ctx) {
   TuneInfo t;
   const DevNet& d = ctx->dnet;
@@ -774,6 +782,7 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->sens_work) (void)hipFree(ctx->sens_work);
   if (ctx->con_buf) (void)hipFree(ctx->con_buf);
   if (ctx->status_scratch) (void)hipFree(ctx->status_scratch);
+  if (ctx->tplan) transport_plan_free(ctx->tplan);
   for (int k = 0; k < kStateRing; ++k)
     if (ctx->ring_done[k]) (void)hipEventDestroy(ctx->ring_done[k]);
   delete ctx;

@@ -238,6 +238,22 @@ class Constraint(C.Structure):
     ]
 
 
+class Connections(C.Structure):
+    _fields_ = [
+        ("nconn", C.c_int64),
+        ("id_up", C.c_void_p),
+        ("id_dn", C.c_void_p),
+        ("nbc", C.c_int64),
+        ("bc_cell", C.c_void_p),
+    ]
+
+
+class TransportCoefs(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in
+                ("area", "velocity", "coef_over_dist", "bc_area", "bc_velocity",
+                 "bc_coef_over_dist", "bc_total")]
+
+
 class Launch(C.Structure):
     _fields_ = [
         ("lanes_per_cell", C.c_int32),

@@ -63,6 +63,9 @@ def load(path: str = LIB_PATH):
     L.pfchem_tune_key.argtypes = [vp, C.c_char_p, C.c_int, C.c_char_p, C.c_int]
     L.pfchem_tune_cache_query.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int,
                                           C.c_int64, vp]
+    L.pfchem_transport_setup.argtypes = [vp, C.c_int64, vp]
+    L.pfchem_transport_step.argtypes = [vp, C.c_int64, vp, vp, C.c_double, C.c_double, C.c_int,
+                                        vp, vp]
     _lib = L
     return L
 
@@ -323,6 +326,35 @@ class Context:
                                            flags, C.byref(c), sp), "pfchem_autotune")
         return dict(lanes_per_cell=c.lanes_per_cell, rows_per_lane=c.rows_per_lane,
                     block_size=c.block_size, blocks_per_cu=c.blocks_per_cu)
+
+    def transport_setup(self, ncell, id_up, id_dn, bc_cell):
+        """pfchem_transport_setup: the connection topology (host arrays)"""
+        up = np.ascontiguousarray(id_up, dtype=np.int64)
+        dn = np.ascontiguousarray(id_dn, dtype=np.int64)
+        bc = np.ascontiguousarray(bc_cell, dtype=np.int64)
+        c = abi.Connections(up.size, up.ctypes.data if up.size else None,
+                            dn.ctypes.data if dn.size else None, bc.size,
+                            bc.ctypes.data if bc.size else None)
+        self._check(self.L.pfchem_transport_setup(self.h, ncell, C.byref(c)),
+                    "pfchem_transport_setup")
+
+    def transport_step(self, st: DeviceState, coefs: dict, dt: float, rtol: float = 1e-12,
+                       max_iterations: int = 1000, stream=None, check: bool = True):
+        """pfchem_transport_step: one implicit transport step of the state's
+        totals; coefs: device (torch) arrays area, velocity, coef_over_dist,
+        bc_area, bc_velocity, bc_coef_over_dist, bc_total [naq, nbc].
+        Returns (iterations, return code)"""
+        s = st.to_abi()
+        ptr = lambda k: coefs[k].data_ptr() if coefs.get(k) is not None and coefs[k].numel() else None
+        tc = abi.TransportCoefs(*[ptr(k) for k in ("area", "velocity", "coef_over_dist", "bc_area",
+                                                   "bc_velocity", "bc_coef_over_dist",
+                                                   "bc_total")])
+        it = C.c_int32(0)
+        rc = self.L.pfchem_transport_step(self.h, st.ncell, C.byref(s), C.byref(tc), dt, rtol,
+                                          max_iterations, C.byref(it), _stream_ptr(stream))
+        if check:
+            self._check(rc, "pfchem_transport_step")
+        return it.value, rc
 
     def tune_key(self):
         """(key, bucket) of this context in a tuner cache"""
