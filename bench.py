@@ -598,7 +598,95 @@ def run_configs(args):
         plain_steps=1, steps=2, oracle=variable_oracle)
     # C3: SomDec decomposition, 500,000 columns x 15 layers
     leg("C3_somdec_7500000", workloads.somdec_workload(), 32768, allv, steps=2, band=True)
+    res["osrt_transport"] = transport_leg(args, nthr)
     return res
+
+
+def transport_leg(args, nthr):
+    """the OSRT transport step before the reaction step
+    (pfchem_transport_step) on a 100 x 100 x 100 box of the 15-primary
+    network's components, then the reaction step on the transported totals:
+    times, BiCGSTAB iterations, and parity of the transport against the
+    transport oracle (direct sparse solve) on a 24^3 box"""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from pfchem import abi, workloads
+    from pfchem.lib import Context, DeviceState
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import transport_util as TU
+    t0 = time.time()
+    out = {"workload": "100x100x100 box, 15 aqueous components, Darcy flux 1e-6 m/s along x "
+                       "with a random part on every face, inflow (west) / outflow (east) faces, "
+                       "dt 3600 s; 24^3 box for parity"}
+    # parity against the oracle
+    wl = workloads.random_network_workload(ncell=8)
+    c = TU.box_case(24, 24, 24, wl.net.naqcomp, seed=11)
+    t1 = time.perf_counter()
+    ref = TU.oracle_step(c)
+    t_cpu = time.perf_counter() - t1
+
+    def state_of(c, net):
+        st = wl.state.__class__(net, c["ncell"])
+        n = c["ncell"]
+        st["total"][:, :n] = c["total"]
+        st["porosity"][0, :n] = c["por"]
+        st["saturation"][0, :n] = c["sat"]
+        st["volume"][0, :n] = c["vol"]
+        return st
+    ctx = Context(wl.net, wl.opt)
+    ctx.transport_setup(c["ncell"], c["id_up"], c["id_dn"], c["bc_cell"])
+    ds = DeviceState(state_of(c, wl.net))
+    ctx.transport_step(ds, TU.device_coefs(c), c["dt"], rtol=1e-13)
+    got = ds.to_host()["total"][:, :c["ncell"]]
+    out["parity_24cubed_max_err_rel_to_component_max"] = float(
+        (np.abs(got - ref) / np.abs(ref).max(axis=1, keepdims=True)).max())
+    out["oracle_24cubed_s"] = t_cpu
+    # timing on the 1M-cell box
+    big = TU.box_case(100, 100, 100, wl.net.naqcomp, seed=12)
+    wlb = workloads.random_network_workload(ncell=big["ncell"])
+    st = wlb.state.copy()
+    n = big["ncell"]
+    st["porosity"][0, :n] = big["por"]
+    st["saturation"][0, :n] = big["sat"]
+    st["volume"][0, :n] = big["vol"]
+    # inflow of totals the network speciates: those of the east-face cells
+    nw = big["grid"].face("west")[0].size
+    big["bc_total"][:, :nw] = st["total"][:, big["bc_cell"][nw:]]
+    ctx = Context(wlb.net, wlb.opt)
+    ctx.transport_setup(n, big["id_up"], big["id_dn"], big["bc_cell"])
+    coefs = TU.device_coefs(big)
+    ds0 = DeviceState(st)
+    ctx.transport_step(ds0.clone(), coefs, big["dt"])  # warm-up
+    torch.cuda.synchronize()
+    reps, tt, its = 3, [], []
+    for _ in range(reps):
+        ds = ds0.clone()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        it, _ = ctx.transport_step(ds, coefs, big["dt"])
+        tt.append(time.perf_counter() - t1)
+        its.append(it)
+    ms = 1e3 * min(tt)
+    out["cells"] = n
+    out["transport_ms"] = ms
+    out["transport_cells_per_s"] = n / (ms * 1e-3)
+    out["bicgstab_iterations"] = its
+    # the operator-split step: transport, then the reaction step on its totals
+    for rep in range(2):  # the first pass loads the reaction kernels
+        ds = ds0.clone()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.transport_step(ds, coefs, big["dt"])
+        rc = ctx.L.pfchem_step(ctx.h, n, C.byref(ds.to_abi()), big["dt"],
+                               abi.KERNEL_BATCHED_MIXED, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+    out["osrt_reaction_rc"] = int(rc)
+    out["osrt_step_transport_plus_reaction_ms"] = 1e3 * (time.perf_counter() - t1)
+    out["osrt_failed_cells"] = int((ds.to_host()["status"][0, :n] != 0).sum())
+    out["wall_s"] = time.time() - t0
+    log("[bench] osrt_transport: %s" % json.dumps(out))
+    return out
 
 
 if __name__ == "__main__":

@@ -7,14 +7,30 @@
 // components' vectors side by side) where the reference calls KSPSolve once
 // per component (pmc_subsurface_osrt.F90:286-333).
 //
-// Layout: the matrix is kept as its diagonal plus, per row, the incident
-// connections (column, connection id, role: the row is the connection's
-// upstream or downstream cell), so the values of a step are formed row by
-// row from the connection coefficients without atomics.  Vectors are
-// [component][cell].
+// Layout.  The off-diagonal pattern is stored as sliced ELL with slices of
+// 64 rows, one slice per wavefront: entry (slot j, row 64 s + l) at
+// slice_ptr[s] + 64 j + l, so every load of a slot is one coalesced 512-byte
+// (values) or 256-byte (int32 columns) access and the slice width is
+// uniform in the wave.  Each entry keeps its column and its connection id
+// with the row's role in the sign (id + 1: the row is the connection's
+// upstream cell, -(id + 1): downstream, 0: padding), so the values of a step
+// are formed row by row from the connection coefficients without atomics.
+// Vectors are [component][cell].
+//
+// Solver.  The Jacobi preconditioner is folded into the matrix as a column
+// scaling, A = M D^-1 (unit diagonal), and BiCGSTAB iterates y = D x on
+// A y = b: the iterates of right-preconditioned BiCGSTAB on M x = b, without
+// the preconditioned vectors.  The shadow residual is a hash of (cell,
+// component) recomputed where it is used.  Dot products are summed per
+// block (wave shuffles, then LDS) inside the kernels that produce their
+// operands; one block per component sums the block partials in a fixed
+// order and updates that component's scalars on the device, so the host
+// waits once per iteration (for the components' active flags).  The
+// gathering kernels place consecutive blocks on one XCD (xcd_block).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -26,32 +42,46 @@
 namespace pfchem {
 
 struct TransportPlan {
-  int64_t ncell = 0, nconn = 0, nbc = 0, nnz = 0;
-  // device: row_ptr[ncell + 1], col[nnz], conn[nnz] (role in the sign: conn
-  // id + 1 when the row is the upstream cell, -(id + 1) when downstream),
-  // bc_ptr[ncell + 1], bc_id[nbc]
+  int64_t ncell = 0, nconn = 0, nbc = 0, nslice = 0, nent = 0;  // nent: padded entries
+  int naq = 0;
+  int wmax = 0;  // widest slice (entries per row)
+  // device: slice_ptr[nslice + 1], bc_ptr[ncell + 1] (int64); col and cid
+  // [nent + kEntryPad], bc_id[nbc] (int32)
   void* topo = nullptr;
-  const int64_t* row_ptr = nullptr;
-  const int64_t* col = nullptr;
-  const int64_t* conn = nullptr;
+  const int64_t* slice_ptr = nullptr;
   const int64_t* bc_ptr = nullptr;
-  const int64_t* bc_id = nullptr;
-  // work vectors and reduction partials (grown on demand)
+  const int32_t* col = nullptr;
+  const int32_t* cid = nullptr;
+  const int32_t* bc_id = nullptr;
+  // work vectors, wave partials, scalars and flags (grown on demand)
   double* work = nullptr;
   size_t work_bytes = 0;
+  int32_t* host_flags = nullptr;  // pinned: active and breakdown flags per component
 };
 
 void transport_plan_free(TransportPlan* p) {
   if (!p) return;
   if (p->topo) (void)hipFree(p->topo);
   if (p->work) (void)hipFree(p->work);
+  if (p->host_flags) (void)hipHostFree(p->host_flags);
   delete p;
 }
 
 namespace {
 
-constexpr int kTB = 256;         // threads per block
-constexpr int kMaxPartials = 1024;  // reduction blocks
+constexpr int kTB = 256;      // threads per block: 4 slices
+constexpr int kXcd = 8;       // XCDs of the device
+// Slots held in registers: the kernels take KW in {2, 4, 6, 8}, the
+// smallest that holds the widest slice (wider rows re-read the rest).  The
+// KW register slots are gathered without a test on the slice width (a slot
+// past the row's width holds the cell itself with value 0), so the gathers
+// of a component chunk are issued together instead of one slot at a time.
+constexpr int kMaxKW = 8;
+constexpr int64_t kEntryPad = (int64_t)kMaxKW * 64;  // entries past the last slice
+constexpr int kFinTB = 512;   // threads of the per-component finishing blocks
+constexpr int kSc = 8;        // scalars per component
+constexpr int kMaxNaq = 512;  // components (LDS of the block partials: 3 x 4 x 8 B each)
+constexpr int kRho = 0, kRhoPrev = 1, kAlpha = 2, kOmega = 3, kBeta = 4, kBnorm = 5;
 
 struct Coefs {
   const double *area, *vel, *cod, *bc_area, *bc_vel, *bc_cod, *bc_total;
@@ -70,211 +100,504 @@ __device__ __forceinline__ void flux_coef(double q, double cod, double area, dou
   }
 }
 
-// values of the step's matrix, the right-hand side and the initial guess
-__global__ __launch_bounds__(kTB) void k_assemble(
-    int64_t n, int naq, const int64_t* __restrict__ row_ptr, const int64_t* __restrict__ col,
-    const int64_t* __restrict__ conn, const int64_t* __restrict__ bc_ptr,
-    const int64_t* __restrict__ bc_id, int64_t nbc, Coefs cf, const double* __restrict__ por,
-    const double* __restrict__ sat, const double* __restrict__ vol,
-    const double* __restrict__ total, int64_t ld, double dt, double* __restrict__ diag,
-    double* __restrict__ val, double* __restrict__ b, double* __restrict__ x) {
-  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  const double si = sat[i];
-  const double acc = por[i] * si * 1000.0 * vol[i] / dt;
-  double d = acc;
-  for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
-    const int64_t r = conn[e];
-    const int64_t c = r > 0 ? r - 1 : -r - 1;
-    const int64_t j = col[e];
-    // interior connections check both saturations (check_upwind_saturation)
-    const double s_up = r > 0 ? si : sat[j], s_dn = r > 0 ? sat[j] : si;
-    double t_up = 0.0, t_dn = 0.0;
-    if (!(s_dn < 1.0e-40) && !(s_up < 1.0e-40)) flux_coef(cf.vel[c], cf.cod[c], cf.area[c], t_up, t_dn);
-    if (r > 0) {  // this row is upstream: + T_up on the diagonal, + T_dn off it
-      d += t_up;
-      val[e] = t_dn;
-    } else {  // downstream: - T_dn on the diagonal, - T_up off it
-      d -= t_dn;
-      val[e] = -t_up;
-    }
+// One stored entry of row i: its off-diagonal value, with its part of the
+// diagonal added to d.  rr: connection id + 1 with the row's role in the
+// sign, 0 for padding; c: the other cell.  Interior connections check both
+// saturations (check_upwind_saturation).
+__device__ __forceinline__ double entry_value(int rr, int c, double si,
+                                              const double* __restrict__ sat, const Coefs& cf,
+                                              double& d) {
+  if (rr == 0) return 0.0;
+  const int64_t q = rr > 0 ? rr - 1 : -rr - 1;
+  const double sj = sat[c];
+  const double s_up = rr > 0 ? si : sj, s_dn = rr > 0 ? sj : si;
+  double t_up = 0.0, t_dn = 0.0;
+  if (!(s_dn < 1.0e-40) && !(s_up < 1.0e-40)) flux_coef(cf.vel[q], cf.cod[q], cf.area[q], t_up, t_dn);
+  if (rr > 0) {  // this row is upstream: + T_up on the diagonal, + T_dn off it
+    d += t_up;
+    return t_dn;
   }
-  // a dry cell (no accumulation, no fluxes) has an empty row in the
-  // reference's matrix; here it keeps its totals
-  const bool dry = si < 1.0e-40;
-  diag[i] = dry ? 1.0 : d;
-  for (int k = 0; k < naq; ++k) {
-    if (dry) {
-      b[k * n + i] = total[k * ld + i];
-      x[k * n + i] = total[k * ld + i];
-      continue;
-    }
-    double rb = acc * total[k * ld + i];
-    for (int64_t e = bc_ptr[i]; e < bc_ptr[i + 1]; ++e) {
-      const int64_t q = bc_id[e];
-      if (si < 1.0e-40) continue;
-      double t_up, t_dn;
-      flux_coef(cf.bc_vel[q], cf.bc_cod[q], cf.bc_area[q], t_up, t_dn);
-      rb += t_up * cf.bc_total[k * nbc + q];
-    }
-    b[k * n + i] = rb;
-    x[k * n + i] = total[k * ld + i];
-  }
-  // boundary terms on the diagonal (the cell is the downstream side)
-  if (!(si < 1.0e-40)) {
-    for (int64_t e = bc_ptr[i]; e < bc_ptr[i + 1]; ++e) {
-      const int64_t q = bc_id[e];
-      double t_up, t_dn;
-      flux_coef(cf.bc_vel[q], cf.bc_cod[q], cf.bc_area[q], t_up, t_dn);
-      diag[i] -= t_dn;
-    }
-  }
+  d -= t_dn;  // downstream: - T_dn on the diagonal, - T_up off it
+  return -t_up;
 }
 
-// y_k = M x_k for the active components
-__global__ __launch_bounds__(kTB) void k_spmv(int64_t n, int naq, const int* __restrict__ act,
-                                              const int64_t* __restrict__ row_ptr,
-                                              const int64_t* __restrict__ col,
-                                              const double* __restrict__ diag,
-                                              const double* __restrict__ val,
-                                              const double* __restrict__ x, double* __restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  const int64_t e0 = row_ptr[i], e1 = row_ptr[i + 1];
-  const double d = diag[i];
-  for (int k = 0; k < naq; ++k) {
-    if (!act[k]) continue;
-    const double* xk = x + k * n;
-    double s = d * xk[i];
-    for (int64_t e = e0; e < e1; ++e) s += val[e] * xk[col[e]];
-    y[k * n + i] = s;
-  }
+// the shadow residual: a fixed pseudo-random vector in [0.5, 1.5) (the
+// choice rh = r0 breaks down on upwind advection, whose preconditioned
+// matrix is the identity plus a nilpotent part: the residual moves
+// downstream, orthogonal to r0)
+__device__ __forceinline__ double shadow(int64_t i, int k) {
+  const unsigned h = (unsigned)((unsigned long long)i * 2654435761ull +
+                                (unsigned long long)k * 40503ull);
+  return 0.5 + (double)((h >> 8) & 0xffffu) / 65536.0;
 }
 
-// block partial sums of (a_k . b_k) and (c_k . d_k) (c null: one product)
-__global__ __launch_bounds__(kTB) void k_dot2(int64_t n, int naq, const double* __restrict__ a,
-                                              const double* __restrict__ b,
-                                              const double* __restrict__ c,
-                                              const double* __restrict__ d,
-                                              double* __restrict__ part) {
-  __shared__ double s1[kTB], s2[kTB];
-  const int k = blockIdx.y;
-  const double* ak = a + k * n;
-  const double* bk = b + k * n;
-  double p1 = 0.0, p2 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kTB) {
-    p1 += ak[i] * bk[i];
-    if (c) p2 += c[k * n + i] * d[k * n + i];
+// Block index with consecutive blocks on one XCD: the hardware hands block
+// b to XCD b % 8, so XCD x takes the x-th contiguous run of blocks and the
+// neighbours a row gathers (cells +-1, +-nx, +-nx ny) are mostly its own
+// blocks' cells, in its own L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t g = gridDim.x, b = blockIdx.x;
+  const int64_t per = g / kXcd, rem = g % kXcd, x = b % kXcd, j = b / kXcd;
+  return x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
+}
+
+// Components go through the kernels kKC at a time, so a wave has the
+// loads of kKC components in flight at once and interleaves their
+// reductions (one component at a time leaves every pass latency-bound).
+// on[u]: component k0 + u exists and is active; comp(): the component a
+// lane reads for slot u (k0 for an absent one: valid memory, result unused).
+constexpr int kKC = 4;
+
+__device__ __forceinline__ bool chunk_on(const int32_t* act, int naq, int k0, bool (&on)[kKC]) {
+  bool any = false;
+#pragma unroll
+  for (int u = 0; u < kKC; ++u) {
+    const int k = k0 + u;
+    on[u] = k < naq && (act == nullptr || act[k] != 0);
+    any = any || on[u];
   }
-  s1[threadIdx.x] = p1;
-  s2[threadIdx.x] = p2;
+  return any;
+}
+
+__device__ __forceinline__ int comp(const bool (&on)[kKC], int k0, int u) {
+  return on[u] ? k0 + u : k0;
+}
+
+// wave sums of M values, their shuffle chains interleaved (every lane of
+// the wave takes part)
+template <int M>
+__device__ __forceinline__ void wave_sums(double (&v)[M]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int u = 0; u < M; ++u) v[u] += __shfl_xor(v[u], o, 64);
+}
+
+// lane 0's wave sums of NQ products of the chunk's active components into
+// the block's LDS slots: pr[q kKC + u] -> lds[((q naq + k0 + u) 4 + wave]
+template <int NQ>
+__device__ __forceinline__ void chunk_to_lds(double* lds, int naq, int k0, const bool (&on)[kKC],
+                                             int lane, const double (&pr)[NQ * kKC]) {
+  if (lane != 0) return;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int u = 0; u < kKC; ++u)
+      if (on[u]) lds[((size_t)q * naq + k0 + u) * 4 + (threadIdx.x >> 6)] = pr[q * kKC + u];
+}
+
+// the block's partials of products 0..nq-1 of the active components (after
+// every wave wrote its slots): part[(q naq + k) nblk + blk]
+__device__ __forceinline__ void block_partials(const double* lds, double* part, int nq, int naq,
+                                               const int32_t* act, int64_t blk) {
   __syncthreads();
-  for (int h = kTB / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      s1[threadIdx.x] += s1[threadIdx.x + h];
-      s2[threadIdx.x] += s2[threadIdx.x + h];
+  const int64_t nblk = gridDim.x;
+  for (int t = threadIdx.x; t < nq * naq; t += kTB) {
+    const int k = t % naq;
+    if (act && !act[k]) continue;
+    const double* l = lds + (size_t)t * 4;
+    part[(size_t)t * nblk + blk] = (l[0] + l[1]) + (l[2] + l[3]);
+  }
+}
+
+// Diagonal, off-diagonal values (unscaled), b, the initial iterate
+// y0 = D x0 (x0: the totals) and r0 = b - M x0, with the block partials of
+// (b, b), (r0, r0) and (rh, r0).
+template <int KW>
+__global__ __launch_bounds__(kTB) void k_assemble(
+    int64_t n, int naq, int64_t nslice, const int64_t* __restrict__ sptr,
+    const int32_t* __restrict__ col, const int32_t* __restrict__ cid,
+    const int64_t* __restrict__ bc_ptr, const int32_t* __restrict__ bc_id, int64_t nbc, Coefs cf,
+    const double* __restrict__ por, const double* __restrict__ sat,
+    const double* __restrict__ vol, const double* __restrict__ total, int64_t ld, double dt,
+    double* __restrict__ diag, double* __restrict__ val, double* __restrict__ y,
+    double* __restrict__ r, double* __restrict__ part) {
+  extern __shared__ double lds[];
+  const int64_t blk = xcd_block();
+  const int64_t i = blk * kTB + threadIdx.x;
+  const int64_t sl = i >> 6;
+  const int lane = threadIdx.x & 63;
+  // waves past the last slice (i >= n) take part in the reductions only
+  const bool ok = i < n;
+  const int64_t e0 = sl < nslice ? sptr[sl] : 0;
+  const int w = sl < nslice ? (int)((sptr[sl + 1] - e0) >> 6) : 0;
+  double si = 0.0, acc = 0.0, d = 1.0;
+  double mv[KW];
+  int cv[KW];
+#pragma unroll
+  for (int j = 0; j < KW; ++j) {
+    mv[j] = 0.0;
+    cv[j] = ok ? (int)i : 0;
+  }
+  bool dry = true;
+  if (ok) {
+    si = sat[i];
+    dry = si < 1.0e-40;
+    acc = por[i] * si * 1000.0 * vol[i] / dt;
+    d = acc;
+#pragma unroll
+    for (int j = 0; j < KW; ++j)
+      if (j < w) {
+        const int64_t e = e0 + (int64_t)j * 64 + lane;
+        cv[j] = col[e];
+        mv[j] = entry_value(cid[e], cv[j], si, sat, cf, d);
+        val[e] = mv[j];
+      }
+    for (int j = KW; j < w; ++j) {
+      const int64_t e = e0 + (int64_t)j * 64 + lane;
+      val[e] = entry_value(cid[e], col[e], si, sat, cf, d);
     }
+    // boundary terms on the diagonal (the cell is the downstream side)
+    if (!dry)
+      for (int64_t e = bc_ptr[i]; e < bc_ptr[i + 1]; ++e) {
+        const int q = bc_id[e];
+        double t_up, t_dn;
+        flux_coef(cf.bc_vel[q], cf.bc_cod[q], cf.bc_area[q], t_up, t_dn);
+        d -= t_dn;
+      }
+    // a dry cell (no accumulation, no fluxes) has an empty row in the
+    // reference's matrix; here it keeps its totals
+    if (dry) d = 1.0;
+    diag[i] = d;
+  }
+  for (int k0 = 0; k0 < naq; k0 += kKC) {
+    bool on[kKC];
+    chunk_on(nullptr, naq, k0, on);
+    double pr[3 * kKC];
+#pragma unroll
+    for (int u = 0; u < 3 * kKC; ++u) pr[u] = 0.0;
+    if (ok) {
+      const double* tk[kKC];
+      double x0[kKC], b[kKC], mx[kKC];
+#pragma unroll
+      for (int u = 0; u < kKC; ++u) {
+        tk[u] = total + (size_t)comp(on, k0, u) * ld;
+        x0[u] = tk[u][i];
+        b[u] = x0[u];
+        mx[u] = d * x0[u];
+      }
+      if (!dry) {
+#pragma unroll
+        for (int u = 0; u < kKC; ++u) b[u] = acc * x0[u];
+        for (int64_t e = bc_ptr[i]; e < bc_ptr[i + 1]; ++e) {
+          const int q = bc_id[e];
+          double t_up, t_dn;
+          flux_coef(cf.bc_vel[q], cf.bc_cod[q], cf.bc_area[q], t_up, t_dn);
+#pragma unroll
+          for (int u = 0; u < kKC; ++u) b[u] += t_up * cf.bc_total[(size_t)comp(on, k0, u) * nbc + q];
+        }
+#pragma unroll
+        for (int j = 0; j < KW; ++j)
+#pragma unroll
+          for (int u = 0; u < kKC; ++u) mx[u] += mv[j] * tk[u][cv[j]];
+        for (int j = KW; j < w; ++j) {
+          const int64_t e = e0 + (int64_t)j * 64 + lane;
+          const double m = val[e];
+          const int c = col[e];
+#pragma unroll
+          for (int u = 0; u < kKC; ++u) mx[u] += m * tk[u][c];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kKC; ++u)
+        if (on[u]) {
+          const double res = b[u] - mx[u];
+          y[(size_t)(k0 + u) * n + i] = d * x0[u];
+          r[(size_t)(k0 + u) * n + i] = res;
+          pr[u] = b[u] * b[u];
+          pr[kKC + u] = res * res;
+          pr[2 * kKC + u] = shadow(i, k0 + u) * res;
+        }
+    }
+    wave_sums<3 * kKC>(pr);
+    chunk_to_lds<3>(lds, naq, k0, on, lane, pr);
+  }
+  block_partials(lds, part, 3, naq, nullptr, blk);
+}
+
+// A = M D^-1: every stored value over its column's diagonal
+__global__ __launch_bounds__(kTB) void k_scale(int64_t nent, const int32_t* __restrict__ col,
+                                               const double* __restrict__ diag,
+                                               double* __restrict__ val) {
+  const int64_t e = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (e < nent) val[e] = val[e] / diag[col[e]];
+}
+
+// p = r (first iteration) or r + beta (p - omega v), active components
+__global__ __launch_bounds__(kTB) void k_p(int64_t n, int naq, int first,
+                                           const int32_t* __restrict__ act,
+                                           const double* __restrict__ sc,
+                                           const double* __restrict__ r, double* __restrict__ p,
+                                           const double* __restrict__ v) {
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i >= n) return;
+  for (int k0 = 0; k0 < naq; k0 += kKC) {
+    bool on[kKC];
+    if (!chunk_on(act, naq, k0, on)) continue;  // uniform
+    double rj[kKC], pj[kKC], vj[kKC];
+#pragma unroll
+    for (int u = 0; u < kKC; ++u) {
+      const size_t j = (size_t)comp(on, k0, u) * n + i;
+      rj[u] = r[j];
+      pj[u] = first ? 0.0 : p[j];
+      vj[u] = first ? 0.0 : v[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kKC; ++u)
+      if (on[u]) {
+        const int k = k0 + u;
+        p[(size_t)k * n + i] =
+            first ? rj[u] : rj[u] + sc[k * kSc + kBeta] * (pj[u] - sc[k * kSc + kOmega] * vj[u]);
+      }
+  }
+}
+
+// yv = A xv for the active components, with the block partials of
+// MODE 0: (rh, yv);  MODE 1: (yv, xv) and (yv, yv)
+template <int MODE, int KW>
+__global__ __launch_bounds__(kTB) void k_spmv(int64_t n, int naq, int64_t nslice,
+                                              const int32_t* __restrict__ act,
+                                              const int64_t* __restrict__ sptr,
+                                              const int32_t* __restrict__ col,
+                                              const double* __restrict__ val,
+                                              const double* __restrict__ xv,
+                                              double* __restrict__ yv, double* __restrict__ part) {
+  extern __shared__ double lds[];
+  const int64_t blk = xcd_block();
+  const int64_t i = blk * kTB + threadIdx.x;
+  const int64_t sl = i >> 6;
+  const int lane = threadIdx.x & 63;
+  const bool ok = i < n;
+  const int64_t e0 = sl < nslice ? sptr[sl] : 0;
+  const int w = sl < nslice ? (int)((sptr[sl + 1] - e0) >> 6) : 0;
+  double av[KW];
+  int cv[KW];
+  // the entry arrays hold kMaxKW x 64 entries past the last slice, so the
+  // KW loads stay in bounds; slots past the width are replaced
+#pragma unroll
+  for (int j = 0; j < KW; ++j) {
+    const int64_t e = e0 + (int64_t)j * 64 + lane;
+    const double vv = val[e];
+    const int cc = col[e];
+    av[j] = j < w ? vv : 0.0;
+    cv[j] = j < w ? cc : (ok ? (int)i : 0);
+  }
+  constexpr int NQ = MODE == 0 ? 1 : 2;
+  for (int k0 = 0; k0 < naq; k0 += kKC) {
+    bool on[kKC];
+    if (!chunk_on(act, naq, k0, on)) continue;  // uniform
+    double pr[NQ * kKC];
+#pragma unroll
+    for (int u = 0; u < NQ * kKC; ++u) pr[u] = 0.0;
+    if (ok) {
+      const double* xk[kKC];
+      double xi[kKC], a[kKC];
+#pragma unroll
+      for (int u = 0; u < kKC; ++u) {
+        xk[u] = xv + (size_t)comp(on, k0, u) * n;
+        xi[u] = xk[u][i];
+        a[u] = xi[u];
+      }
+#pragma unroll
+      for (int j = 0; j < KW; ++j)
+#pragma unroll
+        for (int u = 0; u < kKC; ++u) a[u] += av[j] * xk[u][cv[j]];
+      for (int j = KW; j < w; ++j) {
+        const int64_t e = e0 + (int64_t)j * 64 + lane;
+        const double m = val[e];
+        const int c = col[e];
+#pragma unroll
+        for (int u = 0; u < kKC; ++u) a[u] += m * xk[u][c];
+      }
+#pragma unroll
+      for (int u = 0; u < kKC; ++u)
+        if (on[u]) {
+          yv[(size_t)(k0 + u) * n + i] = a[u];
+          if (MODE == 0) {
+            pr[u] = shadow(i, k0 + u) * a[u];
+          } else {
+            pr[u] = a[u] * xi[u];
+            pr[(NQ - 1) * kKC + u] = a[u] * a[u];
+          }
+        }
+    }
+    wave_sums<NQ * kKC>(pr);
+    chunk_to_lds<NQ>(lds, naq, k0, on, lane, pr);
+  }
+  block_partials(lds, part, NQ, naq, act, blk);
+}
+
+// s = r - alpha v, active components
+__global__ __launch_bounds__(kTB) void k_s(int64_t n, int naq, const int32_t* __restrict__ act,
+                                           const double* __restrict__ sc,
+                                           const double* __restrict__ r,
+                                           const double* __restrict__ v, double* __restrict__ s) {
+  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (i >= n) return;
+  for (int k0 = 0; k0 < naq; k0 += kKC) {
+    bool on[kKC];
+    if (!chunk_on(act, naq, k0, on)) continue;  // uniform
+    double rj[kKC], vj[kKC];
+#pragma unroll
+    for (int u = 0; u < kKC; ++u) {
+      const size_t j = (size_t)comp(on, k0, u) * n + i;
+      rj[u] = r[j];
+      vj[u] = v[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kKC; ++u)
+      if (on[u]) {
+        const int k = k0 + u;
+        s[(size_t)k * n + i] = rj[u] - sc[k * kSc + kAlpha] * vj[u];
+      }
+  }
+}
+
+// y += alpha p + omega s; r = s - omega t; block partials of (r, r), (rh, r)
+__global__ __launch_bounds__(kTB) void k_x(int64_t n, int naq,
+                                           const int32_t* __restrict__ act,
+                                           const double* __restrict__ sc,
+                                           const double* __restrict__ p,
+                                           const double* __restrict__ s,
+                                           const double* __restrict__ t, double* __restrict__ y,
+                                           double* __restrict__ r, double* __restrict__ part) {
+  extern __shared__ double lds[];
+  const int64_t i = xcd_block() * kTB + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool ok = i < n;
+  for (int k0 = 0; k0 < naq; k0 += kKC) {
+    bool on[kKC];
+    if (!chunk_on(act, naq, k0, on)) continue;  // uniform
+    double pr[2 * kKC];
+#pragma unroll
+    for (int u = 0; u < 2 * kKC; ++u) pr[u] = 0.0;
+    if (ok) {
+      double pj[kKC], sj[kKC], tj[kKC], yj[kKC];
+#pragma unroll
+      for (int u = 0; u < kKC; ++u) {
+        const size_t j = (size_t)comp(on, k0, u) * n + i;
+        pj[u] = p[j];
+        sj[u] = s[j];
+        tj[u] = t[j];
+        yj[u] = y[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kKC; ++u)
+        if (on[u]) {
+          const int k = k0 + u;
+          const size_t j = (size_t)k * n + i;
+          const double al = sc[k * kSc + kAlpha], om = sc[k * kSc + kOmega];
+          y[j] = yj[u] + (al * pj[u] + om * sj[u]);
+          const double rr = sj[u] - om * tj[u];
+          r[j] = rr;
+          pr[u] = rr * rr;
+          pr[kKC + u] = shadow(i, k) * rr;
+        }
+    }
+    wave_sums<2 * kKC>(pr);
+    chunk_to_lds<2>(lds, naq, k0, on, lane, pr);
+  }
+  block_partials(lds, part, 2, naq, act, xcd_block());
+}
+
+// One block per component: the dot products of a stage (the producing
+// kernels' block partials summed in a fixed order) and that component's
+// scalar update.
+// stage 0 (after assembly): (b,b), (r0,r0), (rh,r0); 1: (rh,v);
+// 2: (t,s), (t,t); 3: (r,r), (rh,r).  flags[k]: active, flags[naq+k]: a
+// breakdown took the component out.
+__global__ __launch_bounds__(kFinTB) void k_fin(int stage, int naq, int64_t nblk,
+                                                const double* __restrict__ part,
+                                                double* __restrict__ sc,
+                                                int32_t* __restrict__ flags, double rtol) {
+  const int k = blockIdx.x;
+  if (stage > 0 && !flags[k]) return;  // uniform in the block
+  const int nq = stage == 0 ? 3 : stage == 1 ? 1 : 2;
+  __shared__ double red[kFinTB];
+  __shared__ double dots[3];
+  for (int q = 0; q < nq; ++q) {
+    const double* pq = part + ((size_t)q * naq + k) * nblk;
+    double s = 0.0;
+    for (int64_t j = threadIdx.x; j < nblk; j += kFinTB) s += pq[j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kFinTB / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) dots[q] = red[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    part[(0 * naq + k) * gridDim.x + blockIdx.x] = s1[0];
-    part[(1 * naq + k) * gridDim.x + blockIdx.x] = s2[0];
+  if (threadIdx.x != 0) return;
+  double* c = sc + k * kSc;
+  int32_t* act = flags + k;
+  int32_t* brk = flags + naq + k;
+  if (stage == 0) {
+    c[kBnorm] = sqrt(dots[0]);
+    c[kRho] = dots[2];
+    c[kRhoPrev] = 1.0;
+    c[kAlpha] = 1.0;
+    c[kOmega] = 1.0;
+    c[kBeta] = 0.0;
+    *act = sqrt(dots[1]) > rtol * c[kBnorm] ? 1 : 0;
+    *brk = 0;
+  } else if (stage == 1) {
+    const double rv = dots[0];
+    if (rv == 0.0 || !isfinite(rv)) {
+      *brk = 1;
+      *act = 0;
+    } else {
+      c[kAlpha] = c[kRho] / rv;
+    }
+  } else if (stage == 2) {
+    c[kOmega] = dots[1] > 0.0 ? dots[0] / dots[1] : 0.0;
+  } else {
+    c[kRhoPrev] = c[kRho];
+    c[kRho] = dots[1];
+    const double rn = sqrt(dots[0]);
+    if (!(rn > rtol * c[kBnorm])) {
+      *act = 0;  // converged
+    } else if (c[kOmega] == 0.0 || c[kRho] == 0.0 || !isfinite(rn)) {
+      *brk = 1;
+      *act = 0;
+    } else {
+      c[kBeta] = (c[kRho] / c[kRhoPrev]) * (c[kAlpha] / c[kOmega]);
+    }
   }
 }
 
-// sums of the partials in a fixed order: out[2 * naq]
-__global__ void k_dot_finish(int naq, int nblk, const double* __restrict__ part,
-                             double* __restrict__ out) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 2 * naq) return;
-  double s = 0.0;
-  for (int j = 0; j < nblk; ++j) s += part[q * nblk + j];
-  out[q] = s;
-}
-
-// r = b - r (r holds M x), p = v = 0, and the shadow residual rh: a fixed
-// pseudo-random vector in [0.5, 1.5) (rh = r0 breaks down on upwind
-// advection, whose preconditioned matrix is the identity plus a nilpotent
-// part: the residual moves downstream, orthogonal to r0)
-__global__ __launch_bounds__(kTB) void k_init(int64_t n, int naq, const double* __restrict__ b,
-                                              double* __restrict__ r, double* __restrict__ rh,
-                                              double* __restrict__ p, double* __restrict__ v) {
-  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  for (int k = 0; k < naq; ++k) {
-    const int64_t j = k * n + i;
-    const double rr = b[j] - r[j];
-    r[j] = rr;
-    const unsigned h = (unsigned)(i * 2654435761ull + (unsigned long long)k * 40503ull);
-    rh[j] = 0.5 + (double)((h >> 8) & 0xffffu) / 65536.0;
-    p[j] = 0.0;
-    v[j] = 0.0;
-  }
-}
-
-// scalars per component: sc[k*4 + 0] beta, 1 omega, 2 alpha
-// p = r + beta (p - omega v); ph = p / diag
-__global__ __launch_bounds__(kTB) void k_p(int64_t n, int naq, const int* __restrict__ act,
-                                           const double* __restrict__ sc,
-                                           const double* __restrict__ diag,
-                                           const double* __restrict__ r, double* __restrict__ p,
-                                           const double* __restrict__ v,
-                                           double* __restrict__ ph) {
-  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  const double id = 1.0 / diag[i];
-  for (int k = 0; k < naq; ++k) {
-    if (!act[k]) continue;
-    const int64_t j = k * n + i;
-    const double pp = r[j] + sc[k * 4 + 0] * (p[j] - sc[k * 4 + 1] * v[j]);
-    p[j] = pp;
-    ph[j] = pp * id;
-  }
-}
-
-// s = r - alpha v; sh = s / diag
-__global__ __launch_bounds__(kTB) void k_s(int64_t n, int naq, const int* __restrict__ act,
-                                           const double* __restrict__ sc,
-                                           const double* __restrict__ diag,
-                                           const double* __restrict__ r,
-                                           const double* __restrict__ v, double* __restrict__ s,
-                                           double* __restrict__ sh) {
-  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  const double id = 1.0 / diag[i];
-  for (int k = 0; k < naq; ++k) {
-    if (!act[k]) continue;
-    const int64_t j = k * n + i;
-    const double ss = r[j] - sc[k * 4 + 2] * v[j];
-    s[j] = ss;
-    sh[j] = ss * id;
-  }
-}
-
-// x += alpha ph + omega sh; r = s - omega t
-__global__ __launch_bounds__(kTB) void k_x(int64_t n, int naq, const int* __restrict__ act,
-                                           const double* __restrict__ sc,
-                                           const double* __restrict__ ph,
-                                           const double* __restrict__ sh,
-                                           const double* __restrict__ s,
-                                           const double* __restrict__ t, double* __restrict__ x,
-                                           double* __restrict__ r) {
-  const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (i >= n) return;
-  for (int k = 0; k < naq; ++k) {
-    if (!act[k]) continue;
-    const int64_t j = k * n + i;
-    x[j] += sc[k * 4 + 2] * ph[j] + sc[k * 4 + 1] * sh[j];
-    r[j] = s[j] - sc[k * 4 + 1] * t[j];
-  }
-}
-
-// totals of the new time level into the state
-__global__ __launch_bounds__(kTB) void k_store(int64_t n, int naq, const double* __restrict__ x,
+// totals of the new time level into the state: x = D^-1 y
+__global__ __launch_bounds__(kTB) void k_store(int64_t n, int naq, const double* __restrict__ y,
+                                               const double* __restrict__ diag,
                                                double* __restrict__ total, int64_t ld) {
   const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (i >= n) return;
-  for (int k = 0; k < naq; ++k) total[k * ld + i] = x[k * n + i];
+  const double id = 1.0 / diag[i];
+  for (int k0 = 0; k0 < naq; k0 += kKC) {
+    bool on[kKC];
+    chunk_on(nullptr, naq, k0, on);
+    double yj[kKC];
+#pragma unroll
+    for (int u = 0; u < kKC; ++u) yj[u] = y[(size_t)comp(on, k0, u) * n + i];
+#pragma unroll
+    for (int u = 0; u < kKC; ++u)
+      if (on[u]) total[(size_t)(k0 + u) * ld + i] = yj[u] * id;
+  }
+}
+
+// register slots for a widest slice of wmax entries per row
+int slots_for(int wmax) { return wmax <= 2 ? 2 : wmax <= 4 ? 4 : wmax <= 6 ? 6 : 8; }
+
+template <int MODE>
+void launch_spmv(int kw, unsigned grid, size_t lds, hipStream_t s, int64_t n, int naq,
+                 int64_t ns, const int32_t* act, const int64_t* sptr, const int32_t* col,
+                 const double* val, const double* xv, double* yv, double* part) {
+  switch (kw) {
+    case 2: k_spmv<MODE, 2><<<grid, kTB, lds, s>>>(n, naq, ns, act, sptr, col, val, xv, yv, part); break;
+    case 4: k_spmv<MODE, 4><<<grid, kTB, lds, s>>>(n, naq, ns, act, sptr, col, val, xv, yv, part); break;
+    case 6: k_spmv<MODE, 6><<<grid, kTB, lds, s>>>(n, naq, ns, act, sptr, col, val, xv, yv, part); break;
+    default: k_spmv<MODE, 8><<<grid, kTB, lds, s>>>(n, naq, ns, act, sptr, col, val, xv, yv, part); break;
+  }
 }
 
 }  // namespace
@@ -288,6 +611,11 @@ extern "C" int pfchem_transport_setup(pfchem_ctx* ctx, int64_t ncell,
       (conn->nconn > 0 && (!conn->id_up || !conn->id_dn)) || (conn->nbc > 0 && !conn->bc_cell))
     return ctx ? ctx_fail(ctx, PFCHEM_ERR_INVALID, "transport setup: bad arguments")
                : PFCHEM_ERR_INVALID;
+  if (ncell >= INT32_MAX || conn->nconn >= INT32_MAX - 1 || conn->nbc >= INT32_MAX)
+    return ctx_fail(ctx, PFCHEM_ERR_UNSUPPORTED,
+                    "transport setup: cells and connections are limited to 2^31 - 2");
+  if (ctx_naq(ctx) > kMaxNaq)
+    return ctx_fail(ctx, PFCHEM_ERR_UNSUPPORTED, "transport setup: more than 512 components");
   for (int64_t c = 0; c < conn->nconn; ++c)
     if (conn->id_up[c] < 0 || conn->id_up[c] >= ncell || conn->id_dn[c] < 0 ||
         conn->id_dn[c] >= ncell || conn->id_up[c] == conn->id_dn[c])
@@ -295,71 +623,92 @@ extern "C" int pfchem_transport_setup(pfchem_ctx* ctx, int64_t ncell,
   for (int64_t b = 0; b < conn->nbc; ++b)
     if (conn->bc_cell[b] < 0 || conn->bc_cell[b] >= ncell)
       return ctx_fail(ctx, PFCHEM_ERR_INVALID, "transport setup: boundary cell out of range");
-  // rows: the incident connections of each cell (upstream role first in
-  // connection order, then downstream), the boundary connections
-  std::vector<int64_t> cnt(ncell + 1, 0), bcnt(ncell + 1, 0);
+  // rows: the incident connections of each cell in connection order (both
+  // roles), in 64-row slices as wide as their longest row
+  std::vector<int64_t> deg(ncell, 0), bcnt(ncell + 1, 0);
   for (int64_t c = 0; c < conn->nconn; ++c) {
-    ++cnt[conn->id_up[c] + 1];
-    ++cnt[conn->id_dn[c] + 1];
+    ++deg[conn->id_up[c]];
+    ++deg[conn->id_dn[c]];
   }
   for (int64_t b = 0; b < conn->nbc; ++b) ++bcnt[conn->bc_cell[b] + 1];
-  for (int64_t i = 0; i < ncell; ++i) {
-    cnt[i + 1] += cnt[i];
-    bcnt[i + 1] += bcnt[i];
+  for (int64_t i = 0; i < ncell; ++i) bcnt[i + 1] += bcnt[i];
+  const int64_t nslice = (ncell + 63) / 64;
+  std::vector<int64_t> sptr(nslice + 1, 0);
+  int64_t wmax = 0;
+  for (int64_t s = 0; s < nslice; ++s) {
+    int64_t w = 0;
+    for (int64_t i = s * 64; i < std::min<int64_t>(ncell, s * 64 + 64); ++i) w = std::max(w, deg[i]);
+    wmax = std::max(wmax, w);
+    sptr[s + 1] = sptr[s] + 64 * w;
   }
-  const int64_t nnz = cnt[ncell];
-  std::vector<int64_t> col(nnz > 0 ? nnz : 1), cid(nnz > 0 ? nnz : 1), fill(cnt.begin(), cnt.end() - 1);
+  const int64_t nent = sptr[nslice];
+  std::vector<int32_t> col(nent + kEntryPad, 0), cid(nent + kEntryPad, 0);
+  // padding: the row itself (a valid cell; the last cell for rows past the
+  // end), connection id 0
+  for (int64_t s = 0; s < nslice; ++s) {
+    const int64_t w = (sptr[s + 1] - sptr[s]) / 64;
+    for (int64_t j = 0; j < w; ++j)
+      for (int l = 0; l < 64; ++l)
+        col[sptr[s] + j * 64 + l] = (int32_t)std::min<int64_t>(s * 64 + l, ncell - 1);
+  }
+  std::vector<int64_t> fill(ncell, 0);
+  auto put = [&](int64_t row, int64_t other, int32_t id) {
+    const int64_t e = sptr[row / 64] + fill[row]++ * 64 + row % 64;
+    col[e] = (int32_t)other;
+    cid[e] = id;
+  };
   for (int64_t c = 0; c < conn->nconn; ++c) {
-    const int64_t u = conn->id_up[c], d = conn->id_dn[c];
-    col[fill[u]] = d;
-    cid[fill[u]++] = c + 1;
-    col[fill[d]] = u;
-    cid[fill[d]++] = -(c + 1);
+    put(conn->id_up[c], conn->id_dn[c], (int32_t)(c + 1));
+    put(conn->id_dn[c], conn->id_up[c], (int32_t)(-(c + 1)));
   }
-  std::vector<int64_t> bid(conn->nbc > 0 ? conn->nbc : 1), bfill(bcnt.begin(), bcnt.end() - 1);
-  for (int64_t b = 0; b < conn->nbc; ++b) bid[bfill[conn->bc_cell[b]]++] = b;
-  // one device buffer: row_ptr, col, conn, bc_ptr, bc_id
-  const size_t n1 = (size_t)ncell + 1;
-  const size_t words = n1 + 2 * (size_t)std::max<int64_t>(nnz, 1) + n1 +
-                       (size_t)std::max<int64_t>(conn->nbc, 1);
+  std::vector<int32_t> bid(std::max<int64_t>(conn->nbc, 1), 0);
+  std::vector<int64_t> bfill(bcnt.begin(), bcnt.end() - 1);
+  for (int64_t b = 0; b < conn->nbc; ++b) bid[bfill[conn->bc_cell[b]]++] = (int32_t)b;
+  // one device buffer: slice_ptr, bc_ptr (int64), col, cid, bc_id (int32)
+  const size_t n64 = (size_t)(nslice + 1) + (size_t)(ncell + 1);
+  const size_t bytes = n64 * 8 + (2 * col.size() + bid.size()) * 4;
   TransportPlan** slot = ctx_transport_plan(ctx);
   if (*slot) {
     transport_plan_free(*slot);
     *slot = nullptr;
   }
   TransportPlan* p = new TransportPlan();
+  p->naq = ctx_naq(ctx);
   hipError_t e = hipSetDevice(ctx_device(ctx));
-  if (e == hipSuccess) e = hipMalloc(&p->topo, words * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc(&p->topo, bytes);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&p->host_flags, 2 * (size_t)p->naq * sizeof(int32_t),
+                      hipHostMallocDefault);
   if (e != hipSuccess) {
-    delete p;
-    return ctx_fail(ctx, PFCHEM_ERR_NOMEM, "transport setup: device memory");
+    transport_plan_free(p);
+    return ctx_fail(ctx, PFCHEM_ERR_NOMEM, "transport setup: memory");
   }
-  std::vector<int64_t> h(words, 0);
-  size_t o = 0;
-  std::memcpy(&h[o], cnt.data(), n1 * sizeof(int64_t));
-  o += n1;
-  std::memcpy(&h[o], col.data(), col.size() * sizeof(int64_t));
-  const size_t ocol = n1, ocid = n1 + std::max<int64_t>(nnz, 1);
-  std::memcpy(&h[ocid], cid.data(), cid.size() * sizeof(int64_t));
-  const size_t obp = ocid + std::max<int64_t>(nnz, 1);
-  std::memcpy(&h[obp], bcnt.data(), n1 * sizeof(int64_t));
-  const size_t obi = obp + n1;
-  std::memcpy(&h[obi], bid.data(), bid.size() * sizeof(int64_t));
-  e = hipMemcpy(p->topo, h.data(), words * sizeof(int64_t), hipMemcpyHostToDevice);
+  std::vector<char> h(bytes);
+  char* o = h.data();
+  std::memcpy(o, sptr.data(), (size_t)(nslice + 1) * 8);
+  std::memcpy(o + (size_t)(nslice + 1) * 8, bcnt.data(), (size_t)(ncell + 1) * 8);
+  char* o32 = o + n64 * 8;
+  std::memcpy(o32, col.data(), col.size() * 4);
+  std::memcpy(o32 + col.size() * 4, cid.data(), cid.size() * 4);
+  std::memcpy(o32 + 2 * col.size() * 4, bid.data(), bid.size() * 4);
+  e = hipMemcpy(p->topo, h.data(), bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     transport_plan_free(p);
     return ctx_fail(ctx, PFCHEM_ERR_HIP, "transport setup: copy");
   }
-  const int64_t* base = (const int64_t*)p->topo;
-  p->row_ptr = base;
-  p->col = base + ocol;
-  p->conn = base + ocid;
-  p->bc_ptr = base + obp;
-  p->bc_id = base + obi;
+  char* base = (char*)p->topo;
+  p->slice_ptr = (const int64_t*)base;
+  p->bc_ptr = (const int64_t*)(base + (size_t)(nslice + 1) * 8);
+  const int32_t* b32 = (const int32_t*)(base + n64 * 8);
+  p->col = b32;
+  p->cid = b32 + col.size();
+  p->bc_id = b32 + 2 * col.size();
   p->ncell = ncell;
   p->nconn = conn->nconn;
   p->nbc = conn->nbc;
-  p->nnz = nnz;
+  p->nslice = nslice;
+  p->nent = nent;
+  p->wmax = (int)std::min<int64_t>(wmax, INT32_MAX);
   *slot = p;
   return PFCHEM_OK;
 }
@@ -382,20 +731,20 @@ extern "C" int pfchem_transport_step(pfchem_ctx* ctx, int64_t ncell, const pfche
     if (iterations) *iterations = 0;
     return PFCHEM_OK;
   }
-  const int naq = ctx_naq(ctx);
+  const int naq = p->naq;
   if (rtol <= 0.0) rtol = 1.0e-12;
   if (max_iterations <= 0) max_iterations = 1000;
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipSetDevice(ctx_device(ctx));
   if (e != hipSuccess) return ctx_fail(ctx, PFCHEM_ERR_HIP, "transport: hipSetDevice");
-  // work: diag, val, 10 vectors of naq x n, partials, dot results,
-  // scalars, active flags
-  const int64_t n = ncell;
-  const int nblk = (int)std::min<int64_t>(kMaxPartials, (n + kTB - 1) / kTB);
+  // work: diag, values, 6 vectors of naq x n (y, r, p, v, s, t), block
+  // partials (3 products), scalars, flags
+  const int64_t n = ncell, ns = p->nslice;
+  const int64_t nblk = (ns + 3) / 4;  // 4 slices per block
   const size_t vec = (size_t)naq * n;
-  const size_t need = ((size_t)n + (size_t)std::max<int64_t>(p->nnz, 1) + 10 * vec +
-                       (size_t)2 * naq * nblk + 2 * naq + 4 * naq) * sizeof(double) +
-                      naq * sizeof(int) + 256;
+  const size_t need = ((size_t)n + (size_t)(p->nent + kEntryPad) + 6 * vec +
+                       (size_t)3 * naq * nblk + (size_t)kSc * naq) * sizeof(double) +
+                      2 * (size_t)naq * sizeof(int32_t) + 256;
   if (need > p->work_bytes) {
     if (p->work) (void)hipFree(p->work);
     p->work = nullptr;
@@ -404,128 +753,72 @@ extern "C" int pfchem_transport_step(pfchem_ctx* ctx, int64_t ncell, const pfche
       return ctx_fail(ctx, PFCHEM_ERR_NOMEM, "transport: work memory");
     p->work_bytes = need;
   }
-  double* w = p->work;
-  double* diag = w;
+  double* diag = p->work;
   double* val = diag + n;
-  double* b = val + std::max<int64_t>(p->nnz, 1);
-  double* x = b + vec;
-  double* r = x + vec;
-  double* rh = r + vec;
-  double* pv = rh + vec;
+  double* y = val + (p->nent + kEntryPad);
+  double* r = y + vec;
+  double* pv = r + vec;
   double* v = pv + vec;
   double* sv = v + vec;
   double* t = sv + vec;
-  double* ph = t + vec;
-  double* sh = ph + vec;
-  double* part = sh + vec;
-  double* dots = part + (size_t)2 * naq * nblk;
-  double* sc = dots + 2 * naq;
-  int* act = (int*)(sc + 4 * naq);
+  double* part = t + vec;
+  double* sc = part + (size_t)3 * naq * nblk;
+  int32_t* flags = (int32_t*)(sc + (size_t)kSc * naq);
 
-  const unsigned grid = (unsigned)((n + kTB - 1) / kTB);
+  const unsigned grid = (unsigned)nblk;
+  const size_t lds = (size_t)3 * naq * 4 * sizeof(double);  // wave sums of <= 3 products
+  const unsigned grid_n = (unsigned)((n + kTB - 1) / kTB);
   Coefs cf{coefs->area, coefs->velocity, coefs->coef_over_dist, coefs->bc_area,
            coefs->bc_velocity, coefs->bc_coef_over_dist, coefs->bc_total};
-  std::vector<int> hact(naq, 1);
-  std::vector<double> hsc(4 * naq, 0.0), hd(2 * naq), rho0_tmp(naq, 0.0);
-  auto upload = [&]() {
-    hipError_t er = hipMemcpyAsync(sc, hsc.data(), 4 * naq * sizeof(double),
-                                   hipMemcpyHostToDevice, s);
-    if (er == hipSuccess)
-      er = hipMemcpyAsync(act, hact.data(), naq * sizeof(int), hipMemcpyHostToDevice, s);
-    return er;
-  };
-  // the two dot products of every component, on the host
-  auto dot2 = [&](const double* a1, const double* b1, const double* c1, const double* d1) {
-    k_dot2<<<dim3(nblk, naq), kTB, 0, s>>>(n, naq, a1, b1, c1, d1, part);
-    k_dot_finish<<<(2 * naq + 63) / 64, 64, 0, s>>>(naq, nblk, part, dots);
-    hipError_t er = hipMemcpyAsync(hd.data(), dots, 2 * naq * sizeof(double),
+  int32_t* hf = p->host_flags;
+  // the components' flags on the host (the one wait of an iteration)
+  auto poll = [&](int& nactive, bool& broke) {
+    hipError_t er = hipMemcpyAsync(hf, flags, 2 * (size_t)naq * sizeof(int32_t),
                                    hipMemcpyDeviceToHost, s);
     if (er == hipSuccess) er = hipStreamSynchronize(s);
+    nactive = 0;
+    for (int k = 0; k < naq; ++k) {
+      nactive += hf[k] != 0;
+      broke = broke || hf[naq + k] != 0;
+    }
     return er;
   };
 
-  k_assemble<<<grid, kTB, 0, s>>>(n, naq, p->row_ptr, p->col, p->conn, p->bc_ptr, p->bc_id,
-                                  p->nbc, cf, state->porosity, state->saturation, state->volume,
-                                  state->total, state->ld, dt, diag, val, b, x);
-  e = upload();
-  if (e == hipSuccess) {
-    k_spmv<<<grid, kTB, 0, s>>>(n, naq, act, p->row_ptr, p->col, diag, val, x, r);
-    k_init<<<grid, kTB, 0, s>>>(n, naq, b, r, rh, pv, v);
-    e = dot2(b, b, r, r);  // ||b||^2, ||r0||^2
-    if (e == hipSuccess) {
-      const std::vector<double> nb(hd.begin(), hd.end());
-      e = dot2(rh, r, nullptr, nullptr);  // rho_0 = (rh, r0)
-      for (int k = 0; k < naq; ++k) {
-        rho0_tmp[k] = hd[k];
-        hd[k] = nb[k];
-        hd[naq + k] = nb[naq + k];
-      }
-    }
+  const int kw = slots_for(p->wmax);
+#define PFCHEM_ASSEMBLE(KW)                                                                     \
+  k_assemble<KW><<<grid, kTB, lds, s>>>(n, naq, ns, p->slice_ptr, p->col, p->cid, p->bc_ptr,   \
+                                        p->bc_id, p->nbc, cf, state->porosity,                 \
+                                        state->saturation, state->volume, state->total,        \
+                                        state->ld, dt, diag, val, y, r, part)
+  switch (kw) {
+    case 2: PFCHEM_ASSEMBLE(2); break;
+    case 4: PFCHEM_ASSEMBLE(4); break;
+    case 6: PFCHEM_ASSEMBLE(6); break;
+    default: PFCHEM_ASSEMBLE(8); break;
   }
-  if (e != hipSuccess) return ctx_fail(ctx, PFCHEM_ERR_HIP, "transport: set-up kernels");
-  std::vector<double> bnorm(naq), rho(naq), rho_prev(naq, 1.0), alpha(naq, 1.0), omega(naq, 1.0);
+#undef PFCHEM_ASSEMBLE
+  if (p->nent > 0)
+    k_scale<<<(unsigned)((p->nent + kTB - 1) / kTB), kTB, 0, s>>>(p->nent, p->col, diag, val);
+  k_fin<<<naq, kFinTB, 0, s>>>(0, naq, nblk, part, sc, flags, rtol);
   int nactive = 0;
-  for (int k = 0; k < naq; ++k) {
-    bnorm[k] = std::sqrt(hd[k]);
-    const double rn = std::sqrt(hd[naq + k]);
-    hact[k] = rn > rtol * bnorm[k] ? 1 : 0;
-    nactive += hact[k];
-    rho[k] = rho0_tmp[k];
-  }
-  int it = 0;
   bool broke = false;
+  e = poll(nactive, broke);
+  if (e != hipSuccess) return ctx_fail(ctx, PFCHEM_ERR_HIP, "transport: set-up kernels");
+  int it = 0;
   while (nactive > 0 && it < max_iterations) {
     ++it;
-    for (int k = 0; k < naq; ++k) {
-      if (!hact[k]) continue;
-      const double beta = it == 1 ? 0.0 : (rho[k] / rho_prev[k]) * (alpha[k] / omega[k]);
-      hsc[k * 4 + 0] = beta;
-      hsc[k * 4 + 1] = omega[k];
-    }
-    if ((e = upload()) != hipSuccess) break;
-    k_p<<<grid, kTB, 0, s>>>(n, naq, act, sc, diag, r, pv, v, ph);
-    k_spmv<<<grid, kTB, 0, s>>>(n, naq, act, p->row_ptr, p->col, diag, val, ph, v);
-    if ((e = dot2(rh, v, nullptr, nullptr)) != hipSuccess) break;
-    for (int k = 0; k < naq; ++k) {
-      if (!hact[k]) continue;
-      if (hd[k] == 0.0 || !std::isfinite(hd[k])) {
-        broke = true;
-        hact[k] = 0;
-        continue;
-      }
-      alpha[k] = rho[k] / hd[k];
-      hsc[k * 4 + 2] = alpha[k];
-    }
-    if ((e = upload()) != hipSuccess) break;
-    k_s<<<grid, kTB, 0, s>>>(n, naq, act, sc, diag, r, v, sv, sh);
-    k_spmv<<<grid, kTB, 0, s>>>(n, naq, act, p->row_ptr, p->col, diag, val, sh, t);
-    if ((e = dot2(t, sv, t, t)) != hipSuccess) break;
-    for (int k = 0; k < naq; ++k) {
-      if (!hact[k]) continue;
-      omega[k] = hd[naq + k] > 0.0 ? hd[k] / hd[naq + k] : 0.0;
-      hsc[k * 4 + 1] = omega[k];
-    }
-    if ((e = upload()) != hipSuccess) break;
-    k_x<<<grid, kTB, 0, s>>>(n, naq, act, sc, ph, sh, sv, t, x, r);
-    if ((e = dot2(r, r, rh, r)) != hipSuccess) break;
-    nactive = 0;
-    for (int k = 0; k < naq; ++k) {
-      if (!hact[k]) continue;
-      rho_prev[k] = rho[k];
-      rho[k] = hd[naq + k];
-      const double rn = std::sqrt(hd[k]);
-      if (!(rn > rtol * bnorm[k])) {
-        hact[k] = 0;  // converged
-      } else if (omega[k] == 0.0 || rho[k] == 0.0 || !std::isfinite(rn)) {
-        broke = true;
-        hact[k] = 0;
-      } else {
-        ++nactive;
-      }
-    }
+    k_p<<<grid_n, kTB, 0, s>>>(n, naq, it == 1 ? 1 : 0, flags, sc, r, pv, v);
+    launch_spmv<0>(kw, grid, lds, s, n, naq, ns, flags, p->slice_ptr, p->col, val, pv, v, part);
+    k_fin<<<naq, kFinTB, 0, s>>>(1, naq, nblk, part, sc, flags, rtol);
+    k_s<<<grid_n, kTB, 0, s>>>(n, naq, flags, sc, r, v, sv);
+    launch_spmv<1>(kw, grid, lds, s, n, naq, ns, flags, p->slice_ptr, p->col, val, sv, t, part);
+    k_fin<<<naq, kFinTB, 0, s>>>(2, naq, nblk, part, sc, flags, rtol);
+    k_x<<<grid, kTB, lds, s>>>(n, naq, flags, sc, pv, sv, t, y, r, part);
+    k_fin<<<naq, kFinTB, 0, s>>>(3, naq, nblk, part, sc, flags, rtol);
+    if ((e = poll(nactive, broke)) != hipSuccess) break;
   }
   if (e != hipSuccess) return ctx_fail(ctx, PFCHEM_ERR_HIP, "transport: solver kernels");
-  k_store<<<grid, kTB, 0, s>>>(n, naq, x, state->total, state->ld);
+  k_store<<<grid_n, kTB, 0, s>>>(n, naq, y, diag, state->total, state->ld);
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return ctx_fail(ctx, PFCHEM_ERR_HIP, "transport: store");
   if (iterations) *iterations = it;
