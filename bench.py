@@ -51,8 +51,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle leg")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the per-config lines (C1, C3, C4, C5)")
-    ap.add_argument("--only", choices=["plain", "fp64", "mixed", "team-fp64", "team-mixed"],
-                    help="time only this kernel of the step (profiling runs) and exit")
+    ap.add_argument("--only", choices=["plain", "fp64", "mixed", "team-fp64", "team-mixed",
+                                       "transport"],
+                    help="time only this kernel of the step, or only the OSRT transport "
+                         "leg (profiling runs), and exit")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (skip pfchem_autotune)")
     ap.add_argument("--tune-cache", default=None,
@@ -76,6 +78,9 @@ def main():
         print("bench.py measures one GPU: the reaction step is cell-local and this build "
               "runs on one MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
+    if args.only == "transport":
+        print(json.dumps({"osrt_transport": transport_leg(args, 0)}), flush=True)
+        return 0
 
     import numpy as np
     import torch
@@ -672,6 +677,15 @@ def transport_leg(args, nthr):
     out["transport_ms"] = ms
     out["transport_cells_per_s"] = n / (ms * 1e-3)
     out["bicgstab_iterations"] = its
+    # algorithmic HBM bytes of the step: V = one vector of every component,
+    # E = the stored entries (2 per connection; 8 B value + 4 B column)
+    V = 8.0 * wlb.net.naqcomp * n
+    E = 2.0 * big["id_up"].size
+    it = its[int(np.argmin(tt))]
+    setup = 3 * V + E * (4 + 4 + 8) + 32.0 * n + E * (4 + 8 + 8)  # assemble + scale
+    per_it = 17 * V + 2 * 12 * E
+    out["algorithmic_bytes_per_step"] = setup + it * per_it + 2 * V
+    out["hbm_gbs_algorithmic"] = out["algorithmic_bytes_per_step"] / (ms * 1e-3) / 1e9
     # the operator-split step: transport, then the reaction step on its totals
     for rep in range(2):  # the first pass loads the reaction kernels
         ds = ds0.clone()

@@ -220,6 +220,27 @@ module pfchem_c
     type(c_ptr) :: free_ion_guess
   end type pfchem_constraint
 
+  ! OSRT transport: connection topology (host arrays, 0-based cells) and the
+  ! per-step coefficients (device arrays); pfchem.h pfchem_connections /
+  ! pfchem_transport_coefs
+  type, bind(C) :: pfchem_connections
+    integer(c_int64_t) :: nconn
+    type(c_ptr) :: id_up
+    type(c_ptr) :: id_dn
+    integer(c_int64_t) :: nbc
+    type(c_ptr) :: bc_cell
+  end type pfchem_connections
+
+  type, bind(C) :: pfchem_transport_coefs
+    type(c_ptr) :: area
+    type(c_ptr) :: velocity
+    type(c_ptr) :: coef_over_dist
+    type(c_ptr) :: bc_area
+    type(c_ptr) :: bc_velocity
+    type(c_ptr) :: bc_coef_over_dist
+    type(c_ptr) :: bc_total
+  end type pfchem_transport_coefs
+
   interface
     integer(c_int) function pfchem_create(net, opt, device, ctx) bind(C, name="pfchem_create")
       import :: c_int, c_ptr, pfchem_network, pfchem_options
@@ -290,6 +311,28 @@ module pfchem_c
       type(c_ptr), value :: conc
       type(c_ptr), value :: stream
     end function pfchem_equilibrate
+
+    integer(c_int) function pfchem_transport_setup(ctx, ncell, conn) &
+        bind(C, name="pfchem_transport_setup")
+      import :: c_int, c_int64_t, c_ptr, pfchem_connections
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ncell
+      type(pfchem_connections), intent(in) :: conn
+    end function pfchem_transport_setup
+
+    integer(c_int) function pfchem_transport_step(ctx, ncell, state, coefs, dt, rtol, &
+        max_iterations, iterations, stream) bind(C, name="pfchem_transport_step")
+      import :: c_int, c_int32_t, c_int64_t, c_double, c_ptr, pfchem_state, &
+        pfchem_transport_coefs
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: ncell
+      type(pfchem_state), intent(in) :: state
+      type(pfchem_transport_coefs), intent(in) :: coefs
+      real(c_double), value :: dt, rtol
+      integer(c_int), value :: max_iterations
+      integer(c_int32_t), intent(out) :: iterations
+      type(c_ptr), value :: stream
+    end function pfchem_transport_step
 
     integer(c_int) function pfchem_set_launch(ctx, variant, cfg) bind(C, name="pfchem_set_launch")
       import :: c_int, c_ptr, pfchem_launch

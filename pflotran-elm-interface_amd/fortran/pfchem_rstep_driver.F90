@@ -182,6 +182,8 @@ contains
     type(pfchem_somdec), target :: sd
     type(pfchem_launch), target :: la
     type(pfchem_constraint), target :: con
+    type(pfchem_connections), target :: cn
+    type(pfchem_transport_coefs), target :: tc
     integer(c_intptr_t) :: b
     b = transfer(c_loc(net), b)
     write(*, '(*(g0,:," "))') 'network', c_sizeof(net), &
@@ -204,6 +206,12 @@ contains
     write(*, '(*(g0,:," "))') 'constraint', c_sizeof(con), transfer(c_loc(con%nrxn), b) - b, &
       transfer(c_loc(con%h_ion_id), b) - b, transfer(c_loc(con%max_dlnC), b) - b, &
       transfer(c_loc(con%free_ion_guess), b) - b
+    b = transfer(c_loc(cn), b)
+    write(*, '(*(g0,:," "))') 'connections', c_sizeof(cn), transfer(c_loc(cn%nbc), b) - b, &
+      transfer(c_loc(cn%bc_cell), b) - b
+    b = transfer(c_loc(tc), b)
+    write(*, '(*(g0,:," "))') 'transport_coefs', c_sizeof(tc), &
+      transfer(c_loc(tc%bc_area), b) - b, transfer(c_loc(tc%bc_total), b) - b
   end subroutine print_layout
 
 end program pfchem_rstep_driver

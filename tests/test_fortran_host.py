@@ -44,6 +44,10 @@ int main(void) {
   printf("constraint %zu %zu %zu %zu %zu\n", sizeof(pfchem_constraint),
          offsetof(pfchem_constraint, nrxn), offsetof(pfchem_constraint, h_ion_id),
          offsetof(pfchem_constraint, max_dlnC), offsetof(pfchem_constraint, free_ion_guess));
+  printf("connections %zu %zu %zu\n", sizeof(pfchem_connections),
+         offsetof(pfchem_connections, nbc), offsetof(pfchem_connections, bc_cell));
+  printf("transport_coefs %zu %zu %zu\n", sizeof(pfchem_transport_coefs),
+         offsetof(pfchem_transport_coefs, bc_area), offsetof(pfchem_transport_coefs, bc_total));
   return 0;
 }
 """
