@@ -42,6 +42,14 @@ hipError_t launch_equilibrate(const DevNet& n, const DevNet* dn, const DevConstr
                               hipStream_t stream);
 hipError_t launch_plain_step(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                              double dt, hipStream_t stream);
+// ReactionEquilibrateConstraint over teams of L lanes (one row per lane,
+// pfchem_equil_team.hip); equil_team_lanes returns L, 0 when the network
+// takes the one-cell-per-thread kernel (NEWTON activity, more than 64
+// primaries, team LDS above 64 KiB)
+int equil_team_lanes(const DevNet& n);
+hipError_t launch_equil_team(const DevNet& n, const DevNet* dn, const DevConstraint* dk,
+                             const DevState& s, const double* conc, int64_t ncell,
+                             hipStream_t stream);
 hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState& s, int64_t ncell,
                                hipStream_t stream);
 // GIRT reaction block of one iterate, FP64: res[i*ld+c] and, if jac !=

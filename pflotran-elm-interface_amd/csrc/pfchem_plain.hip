@@ -881,6 +881,7 @@ hipError_t launch_activity_lag(const DevNet& n, const DevNet* dn, const DevState
 hipError_t launch_equilibrate(const DevNet& n, const DevNet* dn, const DevConstraint* dk,
                               const DevState& s, const double* conc, int64_t ncell,
                               hipStream_t stream) {
+  if (equil_team_lanes(n) > 0) return launch_equil_team(n, dn, dk, s, conc, ncell, stream);
   const int block = 128;
   const dim3 grid((unsigned)((ncell + block - 1) / block));
   if (n.naq <= 4)
