@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the per-config lines (C1, C3, C4, C5)")
     ap.add_argument("--only", choices=["plain", "fp64", "mixed", "team-fp64", "team-mixed",
-                                       "transport"],
+                                       "transport", "equilibrate"],
                     help="time only this kernel of the step, or only the OSRT transport "
                          "leg (profiling runs), and exit")
     ap.add_argument("--no-tune", action="store_true",
@@ -92,6 +92,10 @@ def main():
     net, opt, host = wl.net, wl.opt, wl.state
     n = host.ncell
     log("[bench] workload %s: %d cells generated in %.1f s" % (wl.name, n, time.time() - t0))
+    if args.only == "equilibrate":
+        print(json.dumps({"constraint_speciation": equilibrate_leg(net, opt, DeviceState(host))}),
+              flush=True)
+        return 0
 
     ctx = Context(net, opt)
     dev0 = DeviceState(host)
@@ -207,36 +211,7 @@ def main():
     log("[bench] girt residual+jacobian %.2f ms  %.3e cells/s" % (g_ms, n / (g_ms * 1e-3)))
     del gst_, fixed
 
-    # --- constraint speciation of every cell from its own totals
-    # (pfchem_equilibrate, TOTAL constraints, fresh guess each call), with
-    # the reference's default constraint tolerances (1e-12 absolute
-    # residual, 1e-6 relative change) and at most 500 iterations ---
-    import dataclasses
-    from pfchem import constraint as pcon
-    cctx = Context(net, dataclasses.replace(opt, max_residual_tolerance=1e-12,
-                                            max_relative_change_tolerance=1e-6))
-    cst = copies(1)[0]
-    con = pcon.Constraint(types=[abi.CONSTRAINT_TOTAL] * naq, rxn=[-1] * naq,
-                          max_iterations=500)
-    conc = cst.t["total"][:naq].clone().contiguous()
-    cctx.equilibrate(cst, con, conc, check=False)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    reps = 3
-    e0.record()
-    for _ in range(reps):
-        cctx.equilibrate(cst, con, conc, check=False)
-    e1.record()
-    torch.cuda.synchronize()
-    c_ms = e0.elapsed_time(e1) / reps
-    cgot = cst.to_host()
-    equil = {"kernel": "equilibrate_kernel (pfchem_equilibrate)", "ms": c_ms,
-             "cells_per_s": n / (c_ms * 1e-3),
-             "mean_iterations": float(cgot["num_iterations"][0, :n].mean()),
-             "failed_cells": int((cgot["status"][0, :n] != 0).sum())}
-    log("[bench] constraint speciation %.2f ms  %.3e cells/s  %.1f iterations  %d failed" %
-        (c_ms, n / (c_ms * 1e-3), equil["mean_iterations"], equil["failed_cells"]))
-    del cst, conc, cgot, cctx
+    equil = equilibrate_leg(net, opt, copies(1)[0])
 
     got = mix_state.to_host()
     status = got["status"][0, :n]
@@ -605,6 +580,51 @@ def run_configs(args):
     leg("C3_somdec_7500000", workloads.somdec_workload(), 32768, allv, steps=2, band=True)
     res["osrt_transport"] = transport_leg(args, nthr)
     return res
+
+
+def equilibrate_leg(net, opt, cst):
+    """constraint speciation of every cell of the device state cst from its
+    own totals (pfchem_equilibrate, TOTAL constraints, fresh guess each
+    call), with the reference's default constraint tolerances (1e-12
+    absolute residual, 1e-6 relative change) and at most 500 iterations"""
+    import dataclasses
+    import torch
+    from pfchem import abi, constraint as pcon
+    from pfchem.lib import Context
+    naq, n = net.naqcomp, cst.ncell
+    cctx = Context(net, dataclasses.replace(opt, max_residual_tolerance=1e-12,
+                                            max_relative_change_tolerance=1e-6))
+    con = pcon.Constraint(types=[abi.CONSTRAINT_TOTAL] * naq, rxn=[-1] * naq,
+                          max_iterations=500)
+    conc = cst.t["total"][:naq].clone().contiguous()
+    cctx.equilibrate(cst, con, conc, check=False)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        cctx.equilibrate(cst, con, conc, check=False)
+    e1.record()
+    torch.cuda.synchronize()
+    c_ms = e0.elapsed_time(e1) / reps
+    cgot = cst.to_host()
+    lanes = 4
+    while lanes < naq:
+        lanes *= 2
+    equil = {"kernel": "equil_team_kernel<%d> (pfchem_equilibrate; teams of %d lanes, one "
+                       "constraint row per lane)" % (lanes, lanes), "ms": c_ms,
+             "cells_per_s": n / (c_ms * 1e-3),
+             "mean_iterations": float(cgot["num_iterations"][0, :n].mean()),
+             "failed_cells": int((cgot["status"][0, :n] != 0).sum())}
+    # algorithmic HBM bytes per cell: constraint values and density read;
+    # free molalities, totals, activity coefficients, secondary molalities
+    # and coefficients, iterations and status written
+    e_bytes = 8 * (naq + 1 + 3 * naq + 2 * net.neqcplx) + 8
+    equil["bytes_per_cell"] = e_bytes
+    equil["hbm_gbs_algorithmic"] = e_bytes * n / (c_ms * 1e-3) / 1e9
+    log("[bench] constraint speciation %.2f ms  %.3e cells/s  %.1f iterations  %d failed" %
+        (c_ms, n / (c_ms * 1e-3), equil["mean_iterations"], equil["failed_cells"]))
+    return equil
 
 
 def transport_leg(args, nthr):
