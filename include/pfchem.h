@@ -263,6 +263,15 @@ struct pfchem_somdec {
   const int32_t* inhibition_type;
   const double* inhibition_constant;
   const double* inhibition_constant2;
+  /* oxygen, as 1-based species codes (0 = none): aqueous id + 1 (> 0) or
+     -(immobile id + 1) (< 0).  Ox_specid[r]: reaction r's oxidizer of the
+     Monod oxygen response (Ox_response_function 1, rxn%Ox_species,
+     reaction_sandbox_somdec.F90:1335-1339); NULL = none for every
+     reaction.  species_o2: the species whose consumption (equal to the CO2
+     release) is tracked (O2_SPECIES_NAME, :1400-1405, 2176-2186). */
+  const int32_t* Ox_specid;
+  int32_t species_o2;
+  int32_t pad1;
 };
 
 /* ------------------------------------------------------------------ */

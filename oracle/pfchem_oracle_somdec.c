@@ -124,6 +124,29 @@ static double species_conc(const sd_ctx* x, int id) {
   return x->c->immobile[-id - 1];
 }
 
+/* a 1-based species code (aqueous id + 1 > 0, -(immobile id + 1) < 0):
+   SomDec_SpeciesConc's value and the residual row (-1 for code 0) */
+static double code_conc(const sd_ctx* x, int code) {
+  return code > 0 ? x->c->total[code - 1] : x->c->immobile[-code - 1];
+}
+static int code_row(const sd_ctx* x, int code) {
+  return code > 0 ? code - 1 : (code < 0 ? x->naq - code - 1 : -1);
+}
+
+/* Monod oxygen response of reaction r (:2129-2147, 2764-2784):
+   GetAerobicCondition (elm_rspfuncs.F90:282-316) = FuncMonod of the
+   oxidizer, and its derivative in the oxidizer, which the reference adds to
+   d crate / d uc */
+static void ox_response(const sd_ctx* x, int r, double* f, double* df) {
+  const pfchem_somdec* sd = x->sd;
+  *f = 1.0;
+  *df = 0.0;
+  if (sd->Ox_response_function[r] != 1 || !sd->Ox_specid || sd->Ox_specid[r] == 0) return;
+  const double ox = code_conc(x, sd->Ox_specid[r]), k = sd->Ox_half_saturation[r];
+  *f = monod(ox, k, 0);
+  *df = monod(ox, k, 1);
+}
+
 #define MD PFCHEM_SOMDEC_MAX_DOWN
 #define MM PFCHEM_SOMDEC_MAX_MONOD
 #define MI PFCHEM_SOMDEC_MAX_INHIB
@@ -203,10 +226,15 @@ static double react1(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, in
   monod_inhib(x, r, 0, uc, &fmb, &dfmb, &d1, &d2, &d3, &d4, 0.0, 0.0);
   dcrate_uc_duc = dcrate_uc_duc * fmb + crate_uc * dfmb;
   crate_uc = crate_uc * fmb;
-  /* Ox response: only OFF and WFPS (applied to f_w) are carried */
+  double f_ox, df_ox; /* Monod oxygen response (WFPS is applied to f_w) */
+  ox_response(x, r, &f_ox, &df_ox);
+  dcrate_uc_duc = dcrate_uc_duc * f_ox + crate_uc * df_ox;
+  crate_uc = crate_uc * f_ox;
+  const int rox = code_row(x, sd->species_o2);
   const double crate = crate_uc;
   R(x, ruc, crate);
   R(x, rco2, -c_st * crate);
+  if (rox >= 0) R(x, rox, c_st * crate); /* O2 consumed as CO2 is released (:2176-2186) */
   if (sd->upstream_hr_id[r] >= 0) R(x, imm(x, sd->upstream_hr_id[r]), -c_st * crate);
   if (sd->species_id_hr >= 0) R(x, imm(x, sd->species_id_hr), -c_st * crate);
   for (int j = 0; j < sd->n_downstream_pools[r]; ++j) {
@@ -233,6 +261,7 @@ static double react1(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, in
   const double dnh4_duc = dco2_duc * n_st; /* as written (:2273) */
   const double dun_duc = c->sd_upstream_nc[r] * duc_duc;
   J(x, rco2, ruc, -dco2_duc);
+  if (rox >= 0) J(x, rox, ruc, dco2_duc); /* :2300-2308 */
   if (sd->upstream_hr_id[r] >= 0) J(x, imm(x, sd->upstream_hr_id[r]), ruc, -dco2_duc);
   if (sd->species_id_hr >= 0) J(x, imm(x, sd->species_id_hr), ruc, -dco2_duc);
   J(x, ruc, ruc, -duc_duc);
@@ -287,6 +316,11 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
   monod_inhib(x, r, 1, uc, &fmb, &dfmb, &fnh4, &dfnh4, &fno3, &dfno3, c_nh4, c_no3);
   dcrate_uc_duc = dcrate_uc_duc * fmb + crate_uc * dfmb;
   crate_uc = crate_uc * fmb;
+  double f_ox, df_ox; /* Monod oxygen response (:2764-2784) */
+  ox_response(x, r, &f_ox, &df_ox);
+  dcrate_uc_duc = dcrate_uc_duc * f_ox + crate_uc * df_ox;
+  crate_uc = crate_uc * f_ox;
+  const int rox = code_row(x, sd->species_o2);
   double feps, dfeps;
   if (has_nh4) {
     if (sd->x0eps > 0.0)
@@ -333,6 +367,7 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
   const double crate = crate_nh4 + crate_no3;
   R(x, ruc, crate);
   R(x, rco2, -c_st * crate);
+  if (rox >= 0) R(x, rox, c_st * crate); /* :2892-2900 */
   if (sd->upstream_hr_id[r] >= 0) R(x, imm(x, sd->upstream_hr_id[r]), -c_st * crate);
   if (sd->species_id_hr >= 0) R(x, imm(x, sd->species_id_hr), -c_st * crate);
   for (int j = 0; j < sd->n_downstream_pools[r]; ++j) {
@@ -385,6 +420,7 @@ static double react2(sd_ctx* x, int r, double crate_uc, double dcrate_uc_duc, do
   const double dun_dno3 = unc * duc_dno3;
 
   J(x, rco2, ruc, -dco2_duc);
+  if (rox >= 0) J(x, rox, ruc, dco2_duc); /* :3104-3111 */
   if (sd->upstream_hr_id[r] >= 0) J(x, imm(x, sd->upstream_hr_id[r]), ruc, -dco2_duc);
   if (sd->species_id_hr >= 0) J(x, imm(x, sd->species_id_hr), ruc, -dco2_duc);
   J(x, ruc, ruc, -duc_duc);

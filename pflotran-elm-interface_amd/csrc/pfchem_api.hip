@@ -155,7 +155,7 @@ hipError_t release_slot(pfchem_ctx* ctx, int k, hipStream_t s) {
 }
 
 // SomDec descriptor: ids in range; the kernels take totals as free molality
-// times density (no complexes) and do not carry the Monod oxygen response
+// times density (no complexes)
 int check_somdec(const pfchem_network* net, std::string& msg) {
   const pfchem_somdec* sd = net->somdec;
   const int naq = net->naqcomp, nimm = net->nimmobile, nr = sd->nrxn;
@@ -229,10 +229,18 @@ int check_somdec(const pfchem_network* net, std::string& msg) {
         return PFCHEM_ERR_INVALID;
       }
     }
-    if (sd->Ox_response_function[r] == 1) {
-      msg = "somdec: the Monod oxygen response is not supported";
-      return PFCHEM_ERR_UNSUPPORTED;
+    // oxygen codes: aqueous id + 1 or -(immobile id + 1), 0 = none
+    if (sd->Ox_specid) {
+      const int o = sd->Ox_specid[r];
+      if (o > naq || o < -nimm) {
+        msg = "somdec: bad oxidizer species in reaction " + std::to_string(r);
+        return PFCHEM_ERR_INVALID;
+      }
     }
+  }
+  if (sd->species_o2 > naq || sd->species_o2 < -nimm) {
+    msg = "somdec: bad O2 species";
+    return PFCHEM_ERR_INVALID;
   }
   return 0;
 }
@@ -434,7 +442,8 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   const int nr = sdh ? sdh->nrxn : 0;
   const size_t nrd = (size_t)nr * PFCHEM_SOMDEC_MAX_DOWN, nrm = (size_t)nr * PFCHEM_SOMDEC_MAX_MONOD;
   const size_t nri = (size_t)nr * PFCHEM_SOMDEC_MAX_INHIB;
-  size_t so[37];
+  size_t so[38];
+  std::vector<int32_t> ox_none;  // Ox_specid when the descriptor has none
   if (sdh) {
     int q = 0;
     so[q++] = addd(sdh->rate_constant, nr);
@@ -472,6 +481,8 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     so[q++] = addi(sdh->inhibition_type, nri);
     so[q++] = addd(sdh->inhibition_constant, nri);
     so[q++] = addd(sdh->inhibition_constant2, nri);
+    if (!sdh->Ox_specid) ox_none.assign(nr > 0 ? nr : 1, 0);
+    so[q++] = addi(sdh->Ox_specid ? sdh->Ox_specid : ox_none.data(), nr);
   }
 
   {
@@ -489,11 +500,12 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     if (sdh) {
       const double sd_scalars[3] = {sdh->x0eps, sdh->n2o_frac_mineralization,
                                     sdh->inhibition_nh4_no3};
-      const int32_t sd_ids[10] = {sdh->species_id_co2,    sdh->species_id_nh4,
+      const int32_t sd_ids[11] = {sdh->species_id_co2,    sdh->species_id_nh4,
                                   sdh->species_id_no3,    sdh->species_id_n2o,
                                   sdh->species_id_proton, sdh->species_id_hr,
                                   sdh->species_id_nmin,   sdh->species_id_nimm,
-                                  sdh->species_id_nimp,   sdh->species_id_ngasmin};
+                                  sdh->species_id_nimp,   sdh->species_id_ngasmin,
+                                  sdh->species_o2};
       mix(sd_scalars, sizeof(sd_scalars));
       mix(sd_ids, sizeof(sd_ids));
     }
@@ -612,6 +624,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     sd.x0eps = sdh->x0eps;
     sd.n2o_frac = sdh->n2o_frac_mineralization;
     sd.inhib_nh4_no3 = sdh->inhibition_nh4_no3;
+    sd.o2 = sdh->species_o2;
     int q = 0;
     sd.rate_constant = pd(so[q++]);
     sd.rate_decomposition = pd(so[q++]);
@@ -648,6 +661,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     sd.inhibition_type = pi(so[q++]);
     sd.inhibition_constant = pd(so[q++]);
     sd.inhibition_constant2 = pd(so[q++]);
+    sd.Ox_specid = pi(so[q++]);
   }
   ctx->net = *net;
   ctx->opt = *opt;

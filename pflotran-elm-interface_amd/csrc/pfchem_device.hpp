@@ -24,6 +24,7 @@ enum FailReason : int { kOk = 0, kMaxIter = 1, kSingular = 2, kActivity = 3 };
 // SomDec sandbox descriptor as kernels see it (pfchem_somdec, 0-based ids)
 struct DevSomdec {
   int nrxn, co2, nh4, no3, n2o, proton, hr, nmin, nimm, nimp, ngasmin;
+  int o2;  // tracked O2: aqueous id + 1, -(immobile id + 1), 0 = none
   double x0eps, n2o_frac, inhib_nh4_no3;
   const double* rate_constant;
   const double* rate_decomposition;
@@ -60,6 +61,7 @@ struct DevSomdec {
   const int* inhibition_type;
   const double* inhibition_constant;
   const double* inhibition_constant2;
+  const int* Ox_specid;  // [nrxn] oxidizer code (as o2), 0 = none
 };
 
 // Network + options as kernels see them: pointers into one device buffer,

@@ -87,6 +87,34 @@ __device__ inline double sd_species_conc(Env& e, int id) {
   return id >= 0 ? e.total(id) : e.imm(-id - 1);
 }
 
+// species of a 1-based code (aqueous id + 1 > 0, -(immobile id + 1) < 0):
+// its concentration (SomDec_SpeciesConc, :3809-3870: total for aqueous,
+// immobile concentration) and its residual row (-1 for code 0)
+template <class Env>
+__device__ inline double sd_code_conc(Env& e, int code) {
+  return code > 0 ? e.total(code - 1) : e.imm(-code - 1);
+}
+template <class Env>
+__device__ inline int sd_code_row(Env& e, int code) {
+  return code > 0 ? code - 1 : (code < 0 ? e.naq - code - 1 : -1);
+}
+
+// Monod oxygen response of reaction r (GetAerobicCondition,
+// elm_rspfuncs.F90:282-316, at the oxidizer's concentration; :2129-2147,
+// 2764-2784): f = Ox / (K + Ox), and df the derivative in Ox that the
+// reference adds to the rate's upstream-pool derivative
+template <class Env>
+__device__ inline void sd_ox(const DevSomdec& sd, Env& e, int r, double* f, double* df) {
+  *f = 1.0;
+  *df = 0.0;
+  if (gld(sd.Ox_response_function + r) != 1) return;
+  const int code = gld(sd.Ox_specid + r);
+  if (code == 0) return;
+  const double ox = sd_code_conc(e, code), k = gld(sd.Ox_half_saturation + r);
+  *f = sd_monod(ox, k, 0);
+  *df = sd_monod(ox, k, 1);
+}
+
 // general Monod / inhibition factors of reaction r; React2 handles its own
 // NH4+ / NO3- Monod terms (skip = 1)
 template <class Env>
@@ -160,11 +188,17 @@ __device__ double sd_react1(const DevSomdec& sd, Env& e, int r, double crate_uc,
   double fmb = 1.0, dfmb = 0.0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
   sd_monod_inhib(sd, e, r, 0, uc, 0.0, 0.0, &fmb, &dfmb, &d1, &d2, &d3, &d4);
   dcrate = dcrate * fmb + crate_uc * dfmb;
-  const double crate = crate_uc * fmb;
+  double crate = crate_uc * fmb;
+  double f_ox, df_ox;
+  sd_ox(sd, e, r, &f_ox, &df_ox);
+  dcrate = dcrate * f_ox + crate * df_ox;
+  crate = crate * f_ox;
+  const int rox = sd_code_row(e, sd.o2);
   e.R(ruc, crate);
   e.R(sd.co2, -c_st * crate);
   if (uhr >= 0) e.R(e.naq + uhr, -c_st * crate);
   if (sd.hr >= 0) e.R(e.naq + sd.hr, -c_st * crate);
+  if (rox >= 0) e.R(rox, c_st * crate);  // O2 consumed as CO2 is released (:2176-2186)
   for (int j = 0; j < nd; ++j) {
     const int dc = gld(sd.downstream_c_id + r * kSdDown + j);
     if (dc >= 0) e.R(e.naq + dc, -gld(sd.downstream_stoich + r * kSdDown + j) * crate);
@@ -183,6 +217,7 @@ __device__ double sd_react1(const DevSomdec& sd, Env& e, int r, double crate_uc,
     const double dco2 = dcrate * c_st, duc = -1.0 * dcrate;
     const double dnh4 = dco2 * n_st;  // as the reference writes it (:2273)
     e.J(sd.co2, ruc, -dco2);
+    if (rox >= 0) e.J(rox, ruc, dco2);  // (:2300-2308)
     if (uhr >= 0) e.J(e.naq + uhr, ruc, -dco2);
     if (sd.hr >= 0) e.J(e.naq + sd.hr, ruc, -dco2);
     e.J(ruc, ruc, -duc);
@@ -237,6 +272,11 @@ __device__ double sd_react2(const DevSomdec& sd, Env& e, int r, double crate_uc,
   sd_monod_inhib(sd, e, r, 1, uc, c_nh4, c_no3, &fmb, &dfmb, &fnh4, &dfnh4, &fno3, &dfno3);
   dcrate = dcrate * fmb + crate_uc * dfmb;
   crate_uc = crate_uc * fmb;
+  double f_ox, df_ox;
+  sd_ox(sd, e, r, &f_ox, &df_ox);
+  dcrate = dcrate * f_ox + crate_uc * df_ox;
+  crate_uc = crate_uc * f_ox;
+  const int rox = sd_code_row(e, sd.o2);
   double h, dh;
   if (has_nh4) {
     if (sd.x0eps > 0.0) {
@@ -279,6 +319,7 @@ __device__ double sd_react2(const DevSomdec& sd, Env& e, int r, double crate_uc,
   e.R(sd.co2, -c_st * crate);
   if (uhr >= 0) e.R(e.naq + uhr, -c_st * crate);
   if (sd.hr >= 0) e.R(e.naq + sd.hr, -c_st * crate);
+  if (rox >= 0) e.R(rox, c_st * crate);  // (:2892-2900)
   for (int j = 0; j < nd; ++j) {
     const int dc = gld(sd.downstream_c_id + r * kSdDown + j);
     if (dc >= 0) e.R(e.naq + dc, -gld(sd.downstream_stoich + r * kSdDown + j) * crate);
@@ -319,6 +360,7 @@ __device__ double sd_react2(const DevSomdec& sd, Env& e, int r, double crate_uc,
   const double dno3_dno3 = (-1.0 * fno3 * dfin_dno3 + dfno3 * (1.0 - fin)) * crate_uc * n_st;
   // column of the upstream pool
   e.J(sd.co2, ruc, -dco2_duc);
+  if (rox >= 0) e.J(rox, ruc, dco2_duc);  // (:3104-3111)
   if (uhr >= 0) e.J(e.naq + uhr, ruc, -dco2_duc);
   if (sd.hr >= 0) e.J(e.naq + sd.hr, ruc, -dco2_duc);
   e.J(ruc, ruc, -duc_duc);

@@ -167,6 +167,10 @@ module pfchem_c
     type(c_ptr) :: inhibition_type
     type(c_ptr) :: inhibition_constant
     type(c_ptr) :: inhibition_constant2
+    ! oxygen, 1-based codes (aqueous id + 1, -(immobile id + 1), 0 = none)
+    type(c_ptr) :: Ox_specid = c_null_ptr
+    integer(c_int32_t) :: species_o2 = 0
+    integer(c_int32_t) :: pad1 = 0
   end type pfchem_somdec
 
   type, bind(C) :: pfchem_state

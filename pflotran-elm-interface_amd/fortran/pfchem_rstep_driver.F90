@@ -200,7 +200,8 @@ contains
       transfer(c_loc(st%status), b) - b, transfer(c_loc(st%ncomp_active), b) - b
     b = transfer(c_loc(sd), b)
     write(*, '(*(g0,:," "))') 'somdec', c_sizeof(sd), transfer(c_loc(sd%x0eps), b) - b, &
-      transfer(c_loc(sd%rate_constant), b) - b, transfer(c_loc(sd%inhibition_constant2), b) - b
+      transfer(c_loc(sd%rate_constant), b) - b, transfer(c_loc(sd%inhibition_constant2), b) - b, &
+      transfer(c_loc(sd%species_o2), b) - b
     write(*, '(*(g0,:," "))') 'launch', c_sizeof(la)
     b = transfer(c_loc(con), b)
     write(*, '(*(g0,:," "))') 'constraint', c_sizeof(con), transfer(c_loc(con%nrxn), b) - b, &

@@ -116,6 +116,9 @@ class Somdec(C.Structure):
         ("inhibition_type", _ip),
         ("inhibition_constant", _dp),
         ("inhibition_constant2", _dp),
+        ("Ox_specid", _ip),
+        ("species_o2", C.c_int32),
+        ("pad1", C.c_int32),
     ]
 
 

@@ -70,6 +70,7 @@ class Reaction:
     moisture_response_function: int = MOISTURE_OFF
     Ox_response_function: int = OX_OFF
     Ox_half_saturation: float = 0.0
+    Ox_species: Optional[str] = None  # oxidizer of the Monod oxygen response
 
 
 @dataclass
@@ -89,6 +90,7 @@ class SomDecNetwork:
     x0eps: float = 1.0e-20
     n2o_frac_mineralization: float = 0.02
     inhibition_nh4_no3: float = 1.0
+    species_o2: int = 0  # tracked O2: aqueous id + 1, -(immobile id + 1), 0 = none
     arrays: dict = field(default_factory=dict)
 
     def to_abi(self, keep: list) -> abi.Somdec:
@@ -96,7 +98,7 @@ class SomDecNetwork:
         for name in ("nrxn", "species_id_co2", "species_id_nh4", "species_id_no3",
                      "species_id_n2o", "species_id_proton", "species_id_hr", "species_id_nmin",
                      "species_id_nimm", "species_id_nimp", "species_id_ngasmin", "x0eps",
-                     "n2o_frac_mineralization", "inhibition_nh4_no3"):
+                     "n2o_frac_mineralization", "inhibition_nh4_no3", "species_o2"):
             setattr(s, name, getattr(self, name))
         for fname, ftype in abi.Somdec._fields_:
             if fname not in self.arrays:
@@ -115,8 +117,10 @@ class SomDecNetwork:
 def build_somdec(pools: Sequence[Pool], reactions: Sequence[Reaction],
                  primary_names: Sequence[str], immobile_names: Sequence[str],
                  x0eps: float = 1.0e-20, n2o_frac_mineralization: float = 0.02,
-                 inhibition_nh4_no3: float = 1.0) -> SomDecNetwork:
-    """SomDecSetup: species ids, setup-time N:C ratios and stoichiometry"""
+                 inhibition_nh4_no3: float = 1.0,
+                 o2_species: Optional[str] = None) -> SomDecNetwork:
+    """SomDecSetup: species ids, setup-time N:C ratios and stoichiometry;
+    o2_species: O2_SPECIES_NAME (its consumption is tracked)"""
     pid = {n: i for i, n in enumerate(primary_names)}
     iid = {n: i for i, n in enumerate(immobile_names)}
 
@@ -178,6 +182,7 @@ def build_somdec(pools: Sequence[Pool], reactions: Sequence[Reaction],
     ity = arr("inhibition_type", (nr, MI), np.int32, 0)
     ic1 = arr("inhibition_constant", (nr, MI), np.float64, 0.0)
     ic2 = arr("inhibition_constant2", (nr, MI), np.float64, 0.0)
+    oxs = arr("Ox_specid", nr, np.int32, 0)
 
     def spec(name):
         if name in pid:
@@ -185,6 +190,10 @@ def build_somdec(pools: Sequence[Pool], reactions: Sequence[Reaction],
         if name in iid:
             return -iid[name] - 1
         raise ValueError("species %s is neither primary nor immobile" % name)
+
+    def code(name):  # 1-based species code: aqueous id + 1, -(immobile id + 1)
+        s = spec(name)
+        return s + 1 if s >= 0 else s
 
     for r, rx in enumerate(reactions):
         if rx.turnover_time is not None:
@@ -213,6 +222,8 @@ def build_somdec(pools: Sequence[Pool], reactions: Sequence[Reaction],
         tf[r], q10[r], ea[r] = rx.temperature_response_function, rx.Q10, rx.Ea
         mf[r], of[r], oh[r] = rx.moisture_response_function, rx.Ox_response_function, \
             rx.Ox_half_saturation
+        if rx.Ox_species is not None:
+            oxs[r] = code(rx.Ox_species)
         if len(rx.monod) > MM or len(rx.inhibition) > MI:
             raise ValueError("too many Monod / inhibition terms")
         nmo[r] = len(rx.monod)
@@ -252,4 +263,5 @@ def build_somdec(pools: Sequence[Pool], reactions: Sequence[Reaction],
         species_id_nmin=imm_or("Nmin"), species_id_nimm=imm_or("Nimm"),
         species_id_nimp=imm_or("Nimp"), species_id_ngasmin=imm_or("NGASmin"),
         x0eps=x0eps, n2o_frac_mineralization=n2o_frac_mineralization,
-        inhibition_nh4_no3=inhibition_nh4_no3, arrays=A)
+        inhibition_nh4_no3=inhibition_nh4_no3,
+        species_o2=code(o2_species) if o2_species is not None else 0, arrays=A)

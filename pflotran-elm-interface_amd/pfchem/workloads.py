@@ -395,12 +395,16 @@ SOMDEC_IMMOBILE = ["SOM1", "SOM2", "SOM3", "SOM4", "Lit1C", "Lit2C", "Lit3C",
 
 
 def somdec_network(rates=None, som_cn=(12.0, 12.0, 10.0, 10.0), half_saturation=1.0e-4,
-                   temperature_function=0, Q10=2.0) -> ReactionNetwork:
+                   temperature_function=0, Q10=2.0,
+                   oxygen: Optional[float] = None) -> ReactionNetwork:
     """The CLM-CN cascade of the clm_cn3 regression deck
     (regression_tests/ngee/CLMCNplus/clm_cn3.in): litter pools Lit1-3 with
     variable C:N, SOM1-4 with fixed C:N, Monod NH4+/NO3- limitation on the
     litter reactions.  rates: 7 first-order rate constants (1/s) in the
-    deck's reaction order, default the deck's turnover times."""
+    deck's reaction order, default the deck's turnover times.  oxygen: a
+    Monod oxygen response of that half saturation (mol/L) on every reaction
+    through an O2(aq) primary, whose consumption is tracked
+    (O2_SPECIES_NAME); None = no oxygen."""
     from . import somdec as SD
     h, d, y = 3600.0, 86400.0, 365.0 * 86400.0
     turnover = [20 * h, 14 * d, 71 * d, 14 * d, 71 * d, 2 * y, 27.4 * y]
@@ -413,13 +417,16 @@ def somdec_network(rates=None, som_cn=(12.0, 12.0, 10.0, 10.0), half_saturation=
                ("Lit3", [("SOM3", 0.71)], mon), ("SOM1", [("SOM2", 0.72)], []),
                ("SOM2", [("SOM3", 0.54)], []), ("SOM3", [("SOM4", 0.45)], []),
                ("SOM4", [], [])]
+    ox = {} if oxygen is None else dict(Ox_response_function=SD.OX_MONOD,
+                                        Ox_half_saturation=oxygen, Ox_species="O2(aq)")
     rx = [SD.Reaction(up, down, rate_constant=k, monod=list(m),
-                      temperature_response_function=temperature_function, Q10=Q10)
+                      temperature_response_function=temperature_function, Q10=Q10, **ox)
           for (up, down, m), k in zip(cascade, rates)]
-    sd = SD.build_somdec(pools, rx, SOMDEC_PRIMARIES, SOMDEC_IMMOBILE)
-    return ReactionNetwork(primary_species_names=list(SOMDEC_PRIMARIES),
-                           primary_spec_Z=np.zeros(len(SOMDEC_PRIMARIES)),
-                           primary_spec_a0=np.zeros(len(SOMDEC_PRIMARIES)),
+    prim = list(SOMDEC_PRIMARIES) + ([] if oxygen is None else ["O2(aq)"])
+    sd = SD.build_somdec(pools, rx, prim, SOMDEC_IMMOBILE,
+                         o2_species=None if oxygen is None else "O2(aq)")
+    return ReactionNetwork(primary_species_names=prim, primary_spec_Z=np.zeros(len(prim)),
+                           primary_spec_a0=np.zeros(len(prim)),
                            immobile_names=list(SOMDEC_IMMOBILE), somdec=sd)
 
 
@@ -428,24 +435,27 @@ def somdec_options(**kw) -> SolverOptions:
 
 
 def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float = 1800.0,
-                    ld: Optional[int] = None) -> Workload:
+                    ld: Optional[int] = None, oxygen: Optional[float] = None) -> Workload:
     """ELM-CN decomposition columns (500,000 columns x 15 layers by default):
     per-reaction first-order rates drawn U[1e-8, 1e-5] /s, SOM C:N (mass)
     U[8, 30], Monod half-saturation 1e-6 mol/m^3; per cell a temperature
     U[0, 35] C acting through Q10 = 2 and a moisture scalar f_w U[0, 1]
     (the ELM w_scalar input); litter C (mol/m^3) logU[1e-2, 1] with C:N
     (mass) U[20, 80], SOM C logU[1e-3, 1], NH4+ / NO3- logU[1e-7, 1e-4]
-    mol/L, CO2 logU[1e-6, 1e-3] mol/L, N2O 1e-20 mol/L, pH 6."""
+    mol/L, CO2 logU[1e-6, 1e-3] mol/L, N2O 1e-20 mol/L, pH 6; with oxygen
+    (a Monod half saturation, mol/L) an O2(aq) primary logU[2e-4, 1e-3]
+    mol/L limits every reaction and is consumed as CO2 is released."""
     rng = _rng(seed)
     rates = 10.0 ** rng.uniform(-8.0, -5.0, 7)
     som_cn = rng.uniform(8.0, 30.0, 4)
     net = somdec_network(rates=list(rates), som_cn=tuple(som_cn), half_saturation=1.0e-6,
-                         temperature_function=2, Q10=2.0)
+                         temperature_function=2, Q10=2.0, oxygen=oxygen)
     st = CellState(net, ncell, ld, somdec_scalars=True)
     n = ncell
     lu = lambda lo, hi, k: 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), k)
     T = np.vstack([lu(1e-6, 1e-3, n), lu(1e-7, 1e-4, n), lu(1e-7, 1e-4, n),
-                   np.full(n, 1e-20), np.full(n, 1e-6)])
+                   np.full(n, 1e-20), np.full(n, 1e-6)] +
+                  ([] if oxygen is None else [lu(2e-4, 1e-3, n)]))
     st["total"][:, :n] = T
     st["pri_molal"][:, :n] = T          # no complexes, unit density below
     st["pri_act_coef"][:, :n] = 1.0
@@ -463,5 +473,6 @@ def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float =
     st["somdec_f_t"][0, :n] = 1.0
     st["somdec_f_w"][0, :n] = rng.uniform(0.0, 1.0, n)
     notes = ("SomDec ELM-CN cascade (clm_cn3 structure); rates U[1e-8,1e-5]/s per reaction, "
-             "SOM C:N U[8,30], Q10 = 2 with per-cell temperature, per-cell f_w U[0,1]")
+             "SOM C:N U[8,30], Q10 = 2 with per-cell temperature, per-cell f_w U[0,1]" +
+             ("" if oxygen is None else ", Monod O2 response K = %g mol/L with O2 tracking" % oxygen))
     return Workload("somdec_%d" % ncell, net, somdec_options(), st, dt, seed, notes)

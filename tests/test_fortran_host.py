@@ -38,8 +38,9 @@ int main(void) {
          offsetof(pfchem_options, max_relative_change_tolerance));
   printf("state %zu %zu %zu %zu\n", sizeof(pfchem_state), offsetof(pfchem_state, temperature),
          offsetof(pfchem_state, status), offsetof(pfchem_state, ncomp_active));
-  printf("somdec %zu %zu %zu %zu\n", sizeof(pfchem_somdec), offsetof(pfchem_somdec, x0eps),
-         offsetof(pfchem_somdec, rate_constant), offsetof(pfchem_somdec, inhibition_constant2));
+  printf("somdec %zu %zu %zu %zu %zu\n", sizeof(pfchem_somdec), offsetof(pfchem_somdec, x0eps),
+         offsetof(pfchem_somdec, rate_constant), offsetof(pfchem_somdec, inhibition_constant2),
+         offsetof(pfchem_somdec, species_o2));
   printf("launch %zu\n", sizeof(pfchem_launch));
   printf("constraint %zu %zu %zu %zu %zu\n", sizeof(pfchem_constraint),
          offsetof(pfchem_constraint, nrxn), offsetof(pfchem_constraint, h_ion_id),

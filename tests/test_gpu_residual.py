@@ -32,6 +32,8 @@ def _case(which, ncell, seed=7):
         wl = workloads.calcite_workload(ncell)
     elif which == "somdec":
         wl = workloads.somdec_workload(ncell=ncell)
+    elif which == "somdec-oxygen":
+        wl = workloads.somdec_workload(ncell=ncell, oxygen=1.0e-5)
     elif which == "prefactor":
         import pin_util as PU
         wl = PU.prefactor_workload(ncell)
@@ -84,12 +86,12 @@ def _oracle(wl, st, fixed, fimm, cells, dt):
 
 
 @pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048), ("somdec", 1024),
-                                         ("prefactor", 512)])
+                                         ("somdec-oxygen", 512), ("prefactor", 512)])
 def test_residual_jacobian_matches_oracle(which, ncell):
     import torch
     from pfchem.lib import Context, DeviceState
     wl, st, fixed, fimm = _case(which, ncell)
-    dt = wl.dt if which == "somdec" else DT
+    dt = wl.dt if which.startswith("somdec") else DT
     print("case residual", which, ncell, flush=True)
     n = st.ncell
     dry_all = np.where(st["saturation"][0, :n] < 1e-40)[0]

@@ -184,3 +184,28 @@ def test_somdec_dry_and_frozen_cells(columns):
         _check(wl, ref, rec, band, got, "somdec-dry/" + abi.KERNEL_NAMES[variant], state=st)
         np.testing.assert_allclose(got["immobile"][:, : n // 2], st.arrays["immobile"][:, : n // 2],
                                    rtol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def oxygen_columns():
+    wl = workloads.somdec_workload(4096, oxygen=1.0e-5)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    return wl, ref, rec, _oracle_band(wl, ref)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_somdec_oxygen_parity(oxygen_columns, variant):
+    """the Monod oxygen response on every reaction (oxidizer O2(aq), K =
+    1e-5 mol/L) with O2 consumption tracked: the oracle band as above, and
+    total O2 + CO2 unchanged in every cell"""
+    wl, ref, rec, band = oxygen_columns
+    got, _ = G.run_gpu(wl, variant)
+    _check(wl, ref, rec, band, got, "somdec-oxygen/" + abi.KERNEL_NAMES[variant])
+    n = wl.state.ncell
+    names = wl.net.primary_species_names
+    ico2, io2 = names.index("CO2(aq)"), names.index("O2(aq)")
+    s0 = wl.state["total"][ico2, :n] + wl.state["total"][io2, :n]
+    s1 = got["total"][ico2, :n] + got["total"][io2, :n]
+    np.testing.assert_allclose(s1, s0, rtol=1e-12)
+    assert np.all(got["total"][io2, :n] < wl.state["total"][io2, :n])

@@ -237,3 +237,65 @@ def test_q10_response_closed_form(temp, factor):
     got = cc.get("immobile", 1)[0]
     want = c0 / (1.0 + k * factor * dt)
     assert abs(got - want) / want < 1e-13, (temp, got, want)
+
+
+def _ox_cell(ox_code_species, half_sat, ox_value, track=None, k=1.0e-5, c0=10.0):
+    """one fixed-C:N pool (C:N 10) decaying to CO2 + NH4+ under the Monod
+    oxygen response, oxidizer either the aqueous O2(aq) or an immobile OXI
+    pool that nothing consumes"""
+    prim = ["CO2(aq)", "NH4+", "H+", "O2(aq)"]
+    imm = ["SOIL4", "OXI"]
+    rx = SD.Reaction("SOIL4", [], rate_constant=k, Ox_response_function=SD.OX_MONOD,
+                     Ox_half_saturation=half_sat, Ox_species=ox_code_species)
+    o2 = ox_value if ox_code_species == "O2(aq)" else 2.5e-4
+    oxi = ox_value if ox_code_species == "OXI" else 1.0
+    cc = _som_cell(prim, imm, [SD.Pool("SOIL4", 10.0)], [rx],
+                   np.array([1e-6, 1e-6, 1e-7, o2]), [c0, oxi], o2_species=track)
+    return cc
+
+
+@pytest.mark.parametrize("oxi,half_sat", [(1.0, 1.0), (0.1, 1.0), (3.0, 0.5), (1e-3, 2.0)])
+def test_oxygen_monod_closed_form(oxi, half_sat):
+    """GetAerobicCondition's Monod response (elm_rspfuncs.F90:282-316) with an
+    oxidizer that the reaction does not consume scales the first-order rate
+    by f = Ox / (K + Ox): one backward-Euler step leaves C0 / (1 + k f dt)"""
+    k, dt, c0 = 1.0e-5, 3600.0, 10.0
+    cc = _ox_cell("OXI", half_sat, oxi, k=k, c0=c0)
+    err, nit, why = cc.rreact(dt)
+    assert err == 0
+    got = cc.get("immobile", 2)
+    f = oxi / (half_sat + oxi)
+    want = c0 / (1.0 + k * f * dt)
+    assert abs(got[0] - want) / want < 1e-13, (got[0], want)
+    assert got[1] == oxi  # the oxidizer is untouched
+
+
+def test_oxygen_monod_closed_form_catches_a_wrong_half_saturation():
+    """the same step with the half saturation 10% off misses the closed form
+    by far more than the bar above"""
+    k, dt, c0, oxi = 1.0e-5, 3600.0, 10.0, 1.0
+    cc = _ox_cell("OXI", 1.1, oxi, k=k, c0=c0)
+    assert cc.rreact(dt)[0] == 0
+    want = c0 / (1.0 + k * oxi / (1.0 + oxi) * dt)
+    assert abs(cc.get("immobile", 2)[0] - want) / want > 1e-4
+
+
+def test_oxygen_tracking_balances_co2():
+    """with O2_SPECIES_NAME the reaction consumes O2 as it releases CO2
+    (reaction_sandbox_somdec.F90:2176-2186): total O2 + CO2 is unchanged and
+    the CO2 released is the pool's C loss over the pore water volume, with
+    O2 both tracked and the Monod oxidizer.  O2 stays at levels where the
+    reference's iteration converges: its Jacobian adds the response's
+    derivative in O2 (per mol/L) to the pool's column, and below ~1e-4 mol/L
+    with K = 1e-5 that term stalls the Newton iteration of one RReact"""
+    k, dt, c0 = 1.0e-5, 3600.0, 0.01
+    psv = 0.25 * 1.0 * 1000.0 * 1.0  # porosity sat 1000 volume (L)
+    for o2 in (2.5e-4, 1.0e-3):
+        cc = _ox_cell("O2(aq)", 1.0e-5, o2, track="O2(aq)", k=k, c0=c0)
+        t0 = cc.get("total", 4).copy()
+        assert cc.rreact(dt)[0] == 0
+        t1 = cc.get("total", 4)
+        im = cc.get("immobile", 2)
+        np.testing.assert_allclose(t1[0] + t1[3], t0[0] + t0[3], rtol=1e-13)
+        np.testing.assert_allclose((t1[0] - t0[0]) * psv, (c0 - im[0]) * 1.0, rtol=1e-11)
+        assert 0.0 < t1[3] < t0[3]
