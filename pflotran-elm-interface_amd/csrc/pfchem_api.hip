@@ -154,8 +154,7 @@ hipError_t release_slot(pfchem_ctx* ctx, int k, hipStream_t s) {
   return e;
 }
 
-// SomDec descriptor: ids in range; the kernels take totals as free molality
-// times density (no complexes)
+// SomDec descriptor: ids in range
 int check_somdec(const pfchem_network* net, std::string& msg) {
   const pfchem_somdec* sd = net->somdec;
   const int naq = net->naqcomp, nimm = net->nimmobile, nr = sd->nrxn;
@@ -163,10 +162,6 @@ int check_somdec(const pfchem_network* net, std::string& msg) {
   auto im = [&](int id, bool req) { return req ? (id >= 0 && id < nimm) : (id >= -1 && id < nimm); };
   if (nr < 0 || nr > 32) {
     msg = "somdec: 0..32 reactions";
-    return PFCHEM_ERR_UNSUPPORTED;
-  }
-  if (net->neqcplx > 0) {
-    msg = "somdec: networks with aqueous complexes are not supported by the kernels";
     return PFCHEM_ERR_UNSUPPORTED;
   }
   if (naq + nimm > 32) {
@@ -662,6 +657,10 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     sd.inhibition_constant = pd(so[q++]);
     sd.inhibition_constant2 = pd(so[q++]);
     sd.Ox_specid = pi(so[q++]);
+    bool ox = sdh->species_o2 != 0;
+    for (int r = 0; r < nr; ++r)
+      ox = ox || (sdh->Ox_response_function[r] == 1 && sdh->Ox_specid && sdh->Ox_specid[r] != 0);
+    sd.ext = (ox || net->neqcplx > 0) ? 1 : 0;
   }
   ctx->net = *net;
   ctx->opt = *opt;

@@ -25,6 +25,7 @@ enum FailReason : int { kOk = 0, kMaxIter = 1, kSingular = 2, kActivity = 3 };
 struct DevSomdec {
   int nrxn, co2, nh4, no3, n2o, proton, hr, nmin, nimm, nimp, ngasmin;
   int o2;  // tracked O2: aqueous id + 1, -(immobile id + 1), 0 = none
+  int ext;  // the network uses the Monod oxygen response, O2 tracking or complexes
   double x0eps, n2o_frac, inhib_nh4_no3;
   const double* rate_constant;
   const double* rate_decomposition;

@@ -34,23 +34,36 @@ __host__ __device__ constexpr int girt_team_doubles(int NM, int nsd) {
 }
 constexpr int kGirtJ = 6;  // offset of the Jacobian tile, in units of NM
 
-// SomDec accumulator of a GIRT team: species values from the team's LDS,
-// residual terms into the reaction residual, Jacobian terms d res_row /
-// d x_col added as they come (the tile is linear here)
-struct GirtSdEnv {
+// SomDec accumulator of a GIRT team: species values from the team's LDS
+// (totals with the complexes' share, the dtotal entries the sandbox reads
+// from the team-summed complex pairs S), residual terms into the reaction
+// residual, Jacobian terms d res_row / d x_col added as they come (the
+// tile is linear here)
+template <bool X>
+struct GirtSdEnvT {
+  static constexpr bool kExt = X;
   typedef __attribute__((address_space(3))) double lds_t;
   const lds_t* xv;
   const lds_t* lna;
   lds_t* racc;
   lds_t* Jt;
   lds_t* st;
+  const lds_t* ctot;  // the complexes' share of the totals (molal)
   int naq, nrxn, ld;
   double dkl, theta, vol, sat, temp, f_t, f_w;
   bool has_f_t, has_f_w;
-  __device__ double total(int i) const { return xv[i] * dkl; }
+  const DevSomdec* sdp;
+  const lds_t* S;  // dtotal pair sums of the cell's complexes
+  __device__ double total(int i) const {
+    if constexpr (X) return (xv[i] + ctot[i]) * dkl;
+    return xv[i] * dkl;
+  }
   __device__ double imm(int i) const { return xv[naq + i]; }
   __device__ double lnact(int i) const { return lna[i]; }
-  __device__ double dtot(int i, int j) const { return i == j ? dkl : 0.0; }
+  __device__ double dtot(int i, int j) const {
+    if constexpr (X) return sd_dtot_value(*sdp, S, dkl, xv[j], i, j);
+    return i == j ? dkl : 0.0;
+  }
   __device__ void R(int row, double v) {
     __hip_atomic_fetch_add(&racc[row], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
@@ -63,6 +76,7 @@ struct GirtSdEnv {
   __device__ lds_t& dnc(int r, int j) { return st[3 * nrxn + r * kSdDown + j]; }
   static __device__ lds_t* lds(double* p) { return (lds_t*)p; }
 };
+using GirtSdEnv = GirtSdEnvT<false>;
 
 template <int L, bool SD>
 __global__ void __launch_bounds__(256)
@@ -72,7 +86,7 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   constexpr int NM = L;
   const DevNet& n = *np;
   const DevState& s = *sp;
-  const int nsd = SD ? n.sd.nrxn * (3 + kSdDown) : 0;
+  const int nsd = SD ? sd_lds_doubles(n.sd.nrxn) : 0;
   const int TD = girt_team_doubles(NM, nsd);
   extern __shared__ __align__(16) char smem[];
   const int teams = (int)(blockDim.x / L);
@@ -87,6 +101,7 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   double* rfin = tb + 5 * NM;
   double* J = tb + kGirtJ * NM;
   double* sdst = J + NM * (NM + 1);
+  double* sdS = SD ? sdst + n.sd.nrxn * (3 + kSdDown) : nullptr;  // SomDec dtotal pair sums
   int* nrow = (int*)((double*)smem + (size_t)teams * TD);  // unknowns per team's cell
   const int64_t c0 = (int64_t)blockIdx.x * teams;
   const int64_t c = c0 + team;
@@ -134,6 +149,7 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
         for (int j = 0; j < kSdDown; ++j)
           sdst[3 * nr + r * kSdDown + j] = gld(n.sd.downstream_nc + r * kSdDown + j);
       }
+      for (int q = t; q < kSdPairs; q += L) sdS[q] = 0.0;
     }
     team_sync();
 
@@ -152,6 +168,7 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       for (int a = 0; a < ncs; ++a) lnQK += gld(st + a) * lnact[gld(ids + 1 + a)];
       const double mk = exp(lnQK) / gld(s.sec_act_coef + (k * ld + c));
       gst(s.sec_molal + (k * ld + c), mk);
+      if constexpr (SD) sd_pair_sums(n.sd, ids, st, mk, GirtSdEnv::lds(sdS));
       for (int a = 0; a < ncs; ++a) {
         const int ia = gld(ids + 1 + a);
         const double wa = gld(st + a) * mk;
@@ -237,16 +254,29 @@ girt_team_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       // SomDec sandbox (SomDecReact, reaction_sandbox_somdec.F90:1504-1910):
       // reactions r = t, t+L, ...; the N2O emission (lane 0) needs the net N
       // mineralization of all of them
-      GirtSdEnv e{GirtSdEnv::lds(xv), GirtSdEnv::lds(lnact), GirtSdEnv::lds(racc),
-                  GirtSdEnv::lds(J), GirtSdEnv::lds(sdst), naq, n.sd.nrxn, NM + 1, dkl,
-                  por * sat, vol, sat, temp,
-                  s.somdec_f_t ? gld(s.somdec_f_t + c) : 1.0,
-                  s.somdec_f_w ? gld(s.somdec_f_w + c) : 1.0,
-                  s.somdec_f_t != nullptr, s.somdec_f_w != nullptr};
-      double nmin = 0.0;
-      for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, dt, true);
-      nmin = team_sum<L>(nmin);
-      if (t == 0) sd_nemission(n.sd, e, nmin, dt, true);
+      const double f_t = s.somdec_f_t ? gld(s.somdec_f_t + c) : 1.0;
+      const double f_w = s.somdec_f_w ? gld(s.somdec_f_w + c) : 1.0;
+      auto run = [&](auto& e) {
+        double nmin = 0.0;
+        for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, dt, true);
+        nmin = team_sum<L>(nmin);
+        if (t == 0) sd_nemission(n.sd, e, nmin, dt, true);
+      };
+      if (n.sd.ext) {
+        GirtSdEnvT<true> e{GirtSdEnv::lds(xv), GirtSdEnv::lds(lnact), GirtSdEnv::lds(racc),
+                           GirtSdEnv::lds(J), GirtSdEnv::lds(sdst), GirtSdEnv::lds(tot), naq,
+                           n.sd.nrxn, NM + 1, dkl, por * sat, vol, sat, temp, f_t, f_w,
+                           s.somdec_f_t != nullptr, s.somdec_f_w != nullptr, &n.sd,
+                           GirtSdEnv::lds(sdS)};
+        run(e);
+      } else {
+        GirtSdEnvT<false> e{GirtSdEnv::lds(xv), GirtSdEnv::lds(lnact), GirtSdEnv::lds(racc),
+                            GirtSdEnv::lds(J), GirtSdEnv::lds(sdst), GirtSdEnv::lds(tot), naq,
+                            n.sd.nrxn, NM + 1, dkl, por * sat, vol, sat, temp, f_t, f_w,
+                            s.somdec_f_t != nullptr, s.somdec_f_w != nullptr, &n.sd,
+                            GirtSdEnv::lds(sdS)};
+        run(e);
+      }
     }
     team_sync();
     if (i < N) rfin[i] = wet ? r + racc[i] : 0.0;
@@ -278,7 +308,7 @@ template <int L>
 hipError_t launch_girt_L(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                          double dt, const double* fixed_total, const double* fixed_imm,
                          double* res, double* jac, hipStream_t stream) {
-  const int nsd = n.has_somdec ? n.sd.nrxn * (3 + kSdDown) : 0;
+  const int nsd = n.has_somdec ? sd_lds_doubles(n.sd.nrxn) : 0;
   const int team_bytes = 8 * girt_team_doubles(L, nsd);
   int teams = 256 / L;
   while (teams > 1 && (size_t)teams * (team_bytes + 4) > 65536) teams /= 2;

@@ -326,6 +326,7 @@ struct PlainSdState {
 // diagonal only.
 template <int MAXN>
 struct PlainSdEnv {
+  static constexpr bool kExt = true;
   PlainCell<MAXN>& w;
   PlainSdState& sd;
   int naq, N;

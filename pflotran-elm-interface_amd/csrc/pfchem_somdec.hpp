@@ -7,6 +7,12 @@
 //   void R(row, v), J(row, col, v)                  residual / d res / d c
 //   double &unc(r), &dnc(r, j), &mcs(r), &mns(r)    per-cell stoichiometry
 //   int naq                                         first immobile dof
+//   static constexpr bool kExt                      oxygen / complexes
+// Kernels instantiate the sandbox twice: Env::kExt = false is the sandbox
+// without the Monod oxygen response, O2 tracking and aqueous complexes (the
+// code a plain ELM-CN network executes, kept as small as before those were
+// added: the kernels are large enough for their instruction-cache footprint
+// to show), Env::kExt = true carries them; DevSomdec::ext selects.
 // Statements follow SomDecReact (:1504-1910), SomDecReact1 (:1914-2418),
 // SomDecReact2 (:2423-3472) and SomDecNemission (:3477-3640), including the
 // reference's Jacobian entries that are not exact derivatives (they steer
@@ -22,6 +28,18 @@ constexpr int kSdDown = PFCHEM_SOMDEC_MAX_DOWN;
 constexpr int kSdMonod = PFCHEM_SOMDEC_MAX_MONOD;
 constexpr int kSdInhib = PFCHEM_SOMDEC_MAX_INHIB;
 constexpr int kSdMaxRxn = 32;
+constexpr int kSdPairs = 8;  // the dtotal pair sums (see sd_pair_sums)
+// LDS doubles of a team's per-cell SomDec state: the stoichiometry of
+// every reaction (upstream N:C, mineral C and N, downstream N:C) and then
+// the dtotal pair sums
+__host__ __device__ constexpr int sd_lds_doubles(int nrxn) {
+  return nrxn * (3 + kSdDown) + kSdPairs;
+}
+// the same for a kernel level: 2 (oxygen / complexes) keeps the pair sums,
+// 1 only the stoichiometry, 0 nothing
+__host__ __device__ constexpr int sd_lds_doubles(int nrxn, int level) {
+  return level >= 2 ? sd_lds_doubles(nrxn) : level == 1 ? nrxn * (3 + kSdDown) : 0;
+}
 
 // HfunctionSmooth (utility.F90:2542-2597)
 __device__ inline void sd_hsmooth(double x, double x1, double x0, double* H, double* dH) {
@@ -80,6 +98,47 @@ __device__ inline double sd_wfps(double sat) {
   const double a = 1.27 - sat, b = sat - 0.0012;
   if (a <= 0.0 || b <= 0.0) return 0.0;
   return fmin(pow(a / 0.67, 3.1777) * pow(b / 0.5988, 2.84), 1.0);
+}
+
+// The dtotal entries the sandbox reads (React2's NH4+ / NO3- columns and
+// SomDecNemission): rows CO2, NH4+, NO3-, N2O by columns NH4+, NO3-.  With
+// aqueous complexes, dtotal_ij = rho/1000 (delta_ij + S_ij / m_j) with
+// S_ij = sum over complexes of nu_i nu_j m_k; the kernels accumulate these
+// eight S_ij from their complex loops (their Jacobian tiles take other
+// terms before the sandbox runs).
+__device__ inline int sd_dtot_row(const DevSomdec& sd, int i) {
+  if (i < 0) return -1;
+  return i == sd.co2 ? 0 : i == sd.nh4 ? 1 : i == sd.no3 ? 2 : i == sd.n2o ? 3 : -1;
+}
+__device__ inline int sd_dtot_col(const DevSomdec& sd, int j) {
+  if (j < 0) return -1;
+  return j == sd.nh4 ? 0 : j == sd.no3 ? 1 : -1;
+}
+// complex k's share nu_i nu_j m_k of the pairs above, added into the
+// team's LDS sums S[2 a + b] (rows a, columns b)
+template <class P>
+__device__ inline void sd_pair_sums(const DevSomdec& sd, const int* ids, const double* st,
+                                    double mk, P S) {
+  const int ncs = gld(ids);
+  for (int a = 0; a < ncs; ++a) {
+    const int ra = sd_dtot_row(sd, gld(ids + 1 + a));
+    if (ra < 0) continue;
+    const double wa = gld(st + a) * mk;
+    for (int b = 0; b < ncs; ++b) {
+      const int cb = sd_dtot_col(sd, gld(ids + 1 + b));
+      if (cb >= 0)
+        __hip_atomic_fetch_add(&S[2 * ra + cb], wa * gld(st + b), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+}
+// dtotal_ij from the pair sums (i, j among the pairs; else the diagonal)
+template <class P>
+__device__ inline double sd_dtot_value(const DevSomdec& sd, P S, double dkl, double xj, int i,
+                                       int j) {
+  const int a = sd_dtot_row(sd, i), b = sd_dtot_col(sd, j);
+  const double d = i == j ? 1.0 : 0.0;
+  return a >= 0 && b >= 0 ? dkl * (d + S[2 * a + b] / xj) : dkl * d;
 }
 
 template <class Env>
@@ -189,16 +248,20 @@ __device__ double sd_react1(const DevSomdec& sd, Env& e, int r, double crate_uc,
   sd_monod_inhib(sd, e, r, 0, uc, 0.0, 0.0, &fmb, &dfmb, &d1, &d2, &d3, &d4);
   dcrate = dcrate * fmb + crate_uc * dfmb;
   double crate = crate_uc * fmb;
-  double f_ox, df_ox;
-  sd_ox(sd, e, r, &f_ox, &df_ox);
-  dcrate = dcrate * f_ox + crate * df_ox;
-  crate = crate * f_ox;
-  const int rox = sd_code_row(e, sd.o2);
+  int rox = -1;
+  if constexpr (Env::kExt) {
+    double f_ox, df_ox;
+    sd_ox(sd, e, r, &f_ox, &df_ox);
+    dcrate = dcrate * f_ox + crate * df_ox;
+    crate = crate * f_ox;
+    rox = sd_code_row(e, sd.o2);
+  }
   e.R(ruc, crate);
   e.R(sd.co2, -c_st * crate);
   if (uhr >= 0) e.R(e.naq + uhr, -c_st * crate);
   if (sd.hr >= 0) e.R(e.naq + sd.hr, -c_st * crate);
-  if (rox >= 0) e.R(rox, c_st * crate);  // O2 consumed as CO2 is released (:2176-2186)
+  if constexpr (Env::kExt)
+    if (rox >= 0) e.R(rox, c_st * crate);  // O2 consumed as CO2 is released (:2176-2186)
   for (int j = 0; j < nd; ++j) {
     const int dc = gld(sd.downstream_c_id + r * kSdDown + j);
     if (dc >= 0) e.R(e.naq + dc, -gld(sd.downstream_stoich + r * kSdDown + j) * crate);
@@ -217,7 +280,8 @@ __device__ double sd_react1(const DevSomdec& sd, Env& e, int r, double crate_uc,
     const double dco2 = dcrate * c_st, duc = -1.0 * dcrate;
     const double dnh4 = dco2 * n_st;  // as the reference writes it (:2273)
     e.J(sd.co2, ruc, -dco2);
-    if (rox >= 0) e.J(rox, ruc, dco2);  // (:2300-2308)
+    if constexpr (Env::kExt)
+      if (rox >= 0) e.J(rox, ruc, dco2);  // (:2300-2308)
     if (uhr >= 0) e.J(e.naq + uhr, ruc, -dco2);
     if (sd.hr >= 0) e.J(e.naq + sd.hr, ruc, -dco2);
     e.J(ruc, ruc, -duc);
@@ -272,11 +336,14 @@ __device__ double sd_react2(const DevSomdec& sd, Env& e, int r, double crate_uc,
   sd_monod_inhib(sd, e, r, 1, uc, c_nh4, c_no3, &fmb, &dfmb, &fnh4, &dfnh4, &fno3, &dfno3);
   dcrate = dcrate * fmb + crate_uc * dfmb;
   crate_uc = crate_uc * fmb;
-  double f_ox, df_ox;
-  sd_ox(sd, e, r, &f_ox, &df_ox);
-  dcrate = dcrate * f_ox + crate_uc * df_ox;
-  crate_uc = crate_uc * f_ox;
-  const int rox = sd_code_row(e, sd.o2);
+  int rox = -1;
+  if constexpr (Env::kExt) {
+    double f_ox, df_ox;
+    sd_ox(sd, e, r, &f_ox, &df_ox);
+    dcrate = dcrate * f_ox + crate_uc * df_ox;
+    crate_uc = crate_uc * f_ox;
+    rox = sd_code_row(e, sd.o2);
+  }
   double h, dh;
   if (has_nh4) {
     if (sd.x0eps > 0.0) {
@@ -319,7 +386,8 @@ __device__ double sd_react2(const DevSomdec& sd, Env& e, int r, double crate_uc,
   e.R(sd.co2, -c_st * crate);
   if (uhr >= 0) e.R(e.naq + uhr, -c_st * crate);
   if (sd.hr >= 0) e.R(e.naq + sd.hr, -c_st * crate);
-  if (rox >= 0) e.R(rox, c_st * crate);  // (:2892-2900)
+  if constexpr (Env::kExt)
+    if (rox >= 0) e.R(rox, c_st * crate);  // (:2892-2900)
   for (int j = 0; j < nd; ++j) {
     const int dc = gld(sd.downstream_c_id + r * kSdDown + j);
     if (dc >= 0) e.R(e.naq + dc, -gld(sd.downstream_stoich + r * kSdDown + j) * crate);
@@ -360,7 +428,8 @@ __device__ double sd_react2(const DevSomdec& sd, Env& e, int r, double crate_uc,
   const double dno3_dno3 = (-1.0 * fno3 * dfin_dno3 + dfno3 * (1.0 - fin)) * crate_uc * n_st;
   // column of the upstream pool
   e.J(sd.co2, ruc, -dco2_duc);
-  if (rox >= 0) e.J(rox, ruc, dco2_duc);  // (:3104-3111)
+  if constexpr (Env::kExt)
+    if (rox >= 0) e.J(rox, ruc, dco2_duc);  // (:3104-3111)
   if (uhr >= 0) e.J(e.naq + uhr, ruc, -dco2_duc);
   if (sd.hr >= 0) e.J(e.naq + sd.hr, ruc, -dco2_duc);
   e.J(ruc, ruc, -duc_duc);

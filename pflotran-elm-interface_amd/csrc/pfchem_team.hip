@@ -1,6 +1,7 @@
 // pfchem_team.hip -- shape table and dispatch of the batched kernel.
 #include "pfchem_device.hpp"
 #include "pfchem_kernels.hpp"
+#include "pfchem_somdec.hpp"
 
 namespace pfchem {
 
@@ -76,7 +77,7 @@ int team_supports(const DevNet& n, const LaunchCfg& cfg, bool mixed) {
   if (n.variable && !team_has_variable(L, R)) return 0;
   if (cfg.block_size % 64 || cfg.block_size > 256 || cfg.block_size < 64) return 0;
   if ((size_t)(cfg.block_size / L) * team_lds_bytes(L * R, n.nc, n.nm, 8,
-                     n.has_somdec ? n.sd.nrxn * (3 + PFCHEM_SOMDEC_MAX_DOWN) : 0) >
+                     n.has_somdec ? sd_lds_doubles(n.sd.nrxn, n.sd.ext ? 2 : 1) : 0) >
       (size_t)cfg.lds_limit)
     return 0;
   return 1;

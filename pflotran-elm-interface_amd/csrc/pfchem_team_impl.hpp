@@ -384,9 +384,13 @@ __device__ void team_prefactor_jacobian(const DevNet& n, int m, const double* ln
 // SomDec accumulator of a team (see pfchem_somdec.hpp): species values from
 // the team's LDS tile, residual terms into the reaction residual, Jacobian
 // terms d res_row / d c_col into the tile in the scaled log form
-// (times c_col and the row scale).  Totals are free molality times density
-// (networks with the sandbox carry no complexes).
-struct TeamSdEnv {
+// (times c_col and the row scale).  X (the oxygen / complexes build):
+// totals are (free molality + the complexes' sum) times density and the
+// dtotal entries the sandbox reads come from the complex pair sums S;
+// without X, free molality times density and a diagonal dtotal.
+template <bool X>
+struct TeamSdEnvT {
+  static constexpr bool kExt = X;
   // LDS-qualified pointers: sd_reaction is too large to inline, and through
   // generic pointers its loads and FP64 atomic adds would take the FLAT path
   typedef __attribute__((address_space(3))) double lds_t;
@@ -396,13 +400,22 @@ struct TeamSdEnv {
   lds_t* racc;
   lds_t* Jt;
   lds_t* st;
+  const lds_t* ctot;  // the complexes' share of the totals (molal)
   int naq, nrxn, ld;
   double dkl, theta, vol, sat, temp, f_t, f_w;
   bool has_f_t, has_f_w;
-  __device__ double total(int i) const { return xv[i] * dkl; }
+  const DevSomdec* sdp;
+  const lds_t* S;  // dtotal pair sums of the cell's complexes
+  __device__ double total(int i) const {
+    if constexpr (X) return (xv[i] + ctot[i]) * dkl;
+    return xv[i] * dkl;
+  }
   __device__ double imm(int i) const { return xv[naq + i]; }
   __device__ double lnact(int i) const { return lna[i]; }
-  __device__ double dtot(int i, int j) const { return i == j ? dkl : 0.0; }
+  __device__ double dtot(int i, int j) const {
+    if constexpr (X) return sd_dtot_value(*sdp, S, dkl, xv[j], i, j);
+    return i == j ? dkl : 0.0;
+  }
   __device__ void R(int row, double v) {
     __hip_atomic_fetch_add(&racc[row], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
@@ -416,6 +429,7 @@ struct TeamSdEnv {
   __device__ lds_t& dnc(int r, int j) { return st[3 * nrxn + r * kSdDown + j]; }
   static __device__ lds_t* lds(double* p) { return (lds_t*)p; }
 };
+using TeamSdEnv = TeamSdEnvT<false>;
 
 // Crout LU with implicit partial pivoting and back substitution
 // (LUDecomposition1 / LUBackSubstitution, utility.F90:597-735) on the
@@ -501,7 +515,7 @@ enum Outcome : int { kContinue = 0, kConverged = 1, kFailMaxIt = 2, kFailSingula
 // SD: the network carries the SomDec sandbox; VAR: cells carry their own
 // active primary counts (options.variable_ncomp).  Both are instantiated for
 // a few team shapes only, so the other networks' kernels keep their code.
-template <int L, int R, typename JT, bool SD, bool VAR>
+template <int L, int R, typename JT, int SD, bool VAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PFCHEM_TEAM_WAVES_PER_EU)))
 team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
                  unsigned long long* __restrict__ cursors, int nxcd, int rescue_only) {
@@ -517,7 +531,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   const int t = lane & (L - 1);          // lane within team
   const int team_in_wave = lane / L;
   const int wave_in_block = threadIdx.x >> 6;
-  const int nsd = SD ? n.sd.nrxn * (3 + kSdDown) : 0;
+  const int nsd = sd_lds_doubles(n.sd.nrxn, SD);
   const TeamLds lay = team_lds_layout(NM, n.nc, n.nm, 8, nsd);
   char* tb = smem + (size_t)(wave_in_block * (64 / L) + team_in_wave) * lay.bytes;
   double* lnact = (double*)(tb + lay.lnact);
@@ -534,6 +548,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   // on gfx950, FP32 ones are not); JT is the precision of the LU
   double* Jt = (double*)(tb + lay.J);
   double* sdst = (double*)(tb + lay.sd);  // SomDec stoichiometry of the team's cell
+  double* sdS = SD >= 2 ? sdst + n.sd.nrxn * (3 + kSdDown) : nullptr;  // its dtotal pair sums
 
   // aqueous primaries and unknowns of the team's cell: the network's, or
   // with VAR the cell's active count (team-uniform)
@@ -661,7 +676,7 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           area[m] = mnrl_on(m) ? gld(s.mnrl_area + (m * ld + c)) : 0.0;
           rate[m] = 0.0;
         }
-        if constexpr (SD) {
+        if constexpr (SD != 0) {
           // per-cell SomDec stoichiometry starts from the Setup values
           const int nr = n.sd.nrxn;
           for (int r = t; r < nr; r += L) {
@@ -811,6 +826,8 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 #pragma unroll
       for (int j = 0; j < NM; ++j) Jt[i * (NM + 1) + j] = 0.0;
     }
+    if constexpr (SD >= 2)
+      for (int q = t; q < kSdPairs; q += L) sdS[q] = 0.0;
     team_sync();
     // phases B and C: complexes (RTotalAqueous) and kinetic minerals
     // (RKineticMineral) scattered into the LDS tile.  LDS operations of a
@@ -877,6 +894,9 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
           const double z = gld(n.cplx_Z + k);
           sec_I += mk[q] * z * z;
           sec_sum += mk[q];
+          if constexpr (SD >= 2)
+            sd_pair_sums(n.sd, n.cplx_ids + k * (cms + 1), n.cplx_st + k * cms, mk[q],
+                         TeamSdEnv::lds(sdS));
         }
       }
 #pragma unroll
@@ -922,18 +942,32 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
       }
       rate[m] = rt;
     }
-    if (SD && wet) {
+    if (SD != 0 && wet) {
       // SomDec sandbox (SomDecReact, reaction_sandbox_somdec.F90:1504-1910):
       // reactions r = t, t+L, ...; the N2O emission needs the net N
       // mineralization of all of them
-      TeamSdEnv e{TeamSdEnv::lds(xv), TeamSdEnv::lds(lnact), TeamSdEnv::lds(rsv),
-                  TeamSdEnv::lds(racc), TeamSdEnv::lds(Jt), TeamSdEnv::lds(sdst), naq,
-                  n.sd.nrxn, NM + 1, dkl, por * sat, vol, sat, temp, sd_ft, sd_fw,
-                  s.somdec_f_t != nullptr, s.somdec_f_w != nullptr};
-      double nmin = 0.0;
-      for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, tdt, true);
-      nmin = team_sum<L>(nmin);
-      if (t == 0) sd_nemission(n.sd, e, nmin, tdt, true);
+      auto run = [&](auto& e) {
+        double nmin = 0.0;
+        for (int r = t; r < n.sd.nrxn; r += L) nmin += sd_reaction(n.sd, e, r, tdt, true);
+        nmin = team_sum<L>(nmin);
+        if (t == 0) sd_nemission(n.sd, e, nmin, tdt, true);
+      };
+      if constexpr (SD >= 2) {
+        team_sync();  // the complexes' totals and dtotal pairs first: the sandbox reads them
+        TeamSdEnvT<true> e{TeamSdEnv::lds(xv), TeamSdEnv::lds(lnact), TeamSdEnv::lds(rsv),
+                           TeamSdEnv::lds(racc), TeamSdEnv::lds(Jt), TeamSdEnv::lds(sdst),
+                           TeamSdEnv::lds(tot), naq, n.sd.nrxn, NM + 1, dkl, por * sat, vol,
+                           sat, temp, sd_ft, sd_fw, s.somdec_f_t != nullptr,
+                           s.somdec_f_w != nullptr, &n.sd, TeamSdEnv::lds(sdS)};
+        run(e);
+      } else {
+        TeamSdEnvT<false> e{TeamSdEnv::lds(xv), TeamSdEnv::lds(lnact), TeamSdEnv::lds(rsv),
+                            TeamSdEnv::lds(racc), TeamSdEnv::lds(Jt), TeamSdEnv::lds(sdst),
+                            TeamSdEnv::lds(tot), naq, n.sd.nrxn, NM + 1, dkl, por * sat, vol,
+                            sat, temp, sd_ft, sd_fw, s.somdec_f_t != nullptr,
+                            s.somdec_f_w != nullptr, &n.sd, TeamSdEnv::lds(sdS)};
+        run(e);
+      }
     }
     team_sync();
 
@@ -1293,11 +1327,11 @@ __global__ void zero_cursors(unsigned long long* c, int n) {
   if ((int)threadIdx.x < n) c[16 * threadIdx.x] = 0ull;
 }
 
-template <int L, int R, typename JT, bool SD, bool VAR>
+template <int L, int R, typename JT, int SD, bool VAR>
 hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                      double dt, const LaunchCfg& cfg, unsigned long long* cursors,
                      int num_cus, int nxcd, hipStream_t stream, int rescue_only) {
-  const TeamLds lay = team_lds_layout(L * R, n.nc, n.nm, 8, SD ? n.sd.nrxn * (3 + kSdDown) : 0);
+  const TeamLds lay = team_lds_layout(L * R, n.nc, n.nm, 8, sd_lds_doubles(n.sd.nrxn, SD));
   const int block = cfg.block_size;
   const size_t lds = (size_t)(block / L) * lay.bytes;
   int64_t blocks = (int64_t)num_cus * cfg.blocks_per_cu;
@@ -1321,7 +1355,7 @@ hipError_t launch_LR(const DevNet& n, const DevNet* dn, const DevState* ds, int6
 // mixed = 0: FP64 variant; 1: mixed variant (FP32 pass, then an FP64 pass
 // over the cells the FP32 pass handed over); kTeamRescue64: only the FP64
 // pass over cells an earlier pass handed over (status kEscalate)
-template <int L, int R, bool SD, bool VAR>
+template <int L, int R, int SD, bool VAR>
 hipError_t launch_variant(const DevNet& n, const DevNet* dn, const DevState* ds, int64_t ncell,
                           double dt, int mixed, const LaunchCfg& cfg,
                           unsigned long long* cursors, int num_cus, int nxcd,
@@ -1351,16 +1385,22 @@ hipError_t launch_pair(const DevNet& n, const DevNet* dn, const DevState* ds, in
                        hipStream_t stream, bool has_status) {
   if (n.variable) {
     if constexpr (team_has_variable(L, R))
-      return launch_variant<L, R, false, true>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
-                                               nxcd, stream, has_status);
+      return launch_variant<L, R, 0, true>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                           nxcd, stream, has_status);
     return hipErrorInvalidValue;
   }
   if (!n.has_somdec)
-    return launch_variant<L, R, false, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
-                                              nxcd, stream, has_status);
-  if constexpr (team_has_somdec(L, R))
-    return launch_variant<L, R, true, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
-                                             nxcd, stream, has_status);
+    return launch_variant<L, R, 0, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                          nxcd, stream, has_status);
+  // the sandbox: level 2 with the Monod oxygen response, O2 tracking or
+  // complexes, level 1 (a smaller kernel) without them
+  if constexpr (team_has_somdec(L, R)) {
+    if (n.sd.ext)
+      return launch_variant<L, R, 2, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                            nxcd, stream, has_status);
+    return launch_variant<L, R, 1, false>(n, dn, ds, ncell, dt, mixed, cfg, cursors, num_cus,
+                                          nxcd, stream, has_status);
+  }
   return hipErrorInvalidValue;
 }
 

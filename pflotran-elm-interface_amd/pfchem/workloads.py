@@ -396,7 +396,7 @@ SOMDEC_IMMOBILE = ["SOM1", "SOM2", "SOM3", "SOM4", "Lit1C", "Lit2C", "Lit3C",
 
 def somdec_network(rates=None, som_cn=(12.0, 12.0, 10.0, 10.0), half_saturation=1.0e-4,
                    temperature_function=0, Q10=2.0,
-                   oxygen: Optional[float] = None) -> ReactionNetwork:
+                   oxygen: Optional[float] = None, complexes: bool = False) -> ReactionNetwork:
     """The CLM-CN cascade of the clm_cn3 regression deck
     (regression_tests/ngee/CLMCNplus/clm_cn3.in): litter pools Lit1-3 with
     variable C:N, SOM1-4 with fixed C:N, Monod NH4+/NO3- limitation on the
@@ -404,7 +404,10 @@ def somdec_network(rates=None, som_cn=(12.0, 12.0, 10.0, 10.0), half_saturation=
     deck's reaction order, default the deck's turnover times.  oxygen: a
     Monod oxygen response of that half saturation (mol/L) on every reaction
     through an O2(aq) primary, whose consumption is tracked
-    (O2_SPECIES_NAME); None = no oxygen."""
+    (O2_SPECIES_NAME); None = no oxygen.  complexes: two synthetic aqueous
+    ion pairs, NH4NO3(aq) = NH4+ + NO3- (log K -4) and NH4CO2(aq) = NH4+ +
+    CO2(aq) (log K -3.5), so the sandbox's totals and dtotal carry
+    complexation."""
     from . import somdec as SD
     h, d, y = 3600.0, 86400.0, 365.0 * 86400.0
     turnover = [20 * h, 14 * d, 71 * d, 14 * d, 71 * d, 2 * y, 27.4 * y]
@@ -425,9 +428,15 @@ def somdec_network(rates=None, som_cn=(12.0, 12.0, 10.0, 10.0), half_saturation=
     prim = list(SOMDEC_PRIMARIES) + ([] if oxygen is None else ["O2(aq)"])
     sd = SD.build_somdec(pools, rx, prim, SOMDEC_IMMOBILE,
                          o2_species=None if oxygen is None else "O2(aq)")
+    cx = {}
+    if complexes:
+        ip = {name: i for i, name in enumerate(prim)}
+        cx = dict(eqcplx_names=["NH4NO3(aq)", "NH4CO2(aq)"],
+                  eqcplx_species=[[ip["NH4+"], ip["NO3-"]], [ip["NH4+"], ip["CO2(aq)"]]],
+                  eqcplx_stoich=[[1.0, 1.0], [1.0, 1.0]], eqcplx_logK=np.array([-4.0, -3.5]))
     return ReactionNetwork(primary_species_names=prim, primary_spec_Z=np.zeros(len(prim)),
                            primary_spec_a0=np.zeros(len(prim)),
-                           immobile_names=list(SOMDEC_IMMOBILE), somdec=sd)
+                           immobile_names=list(SOMDEC_IMMOBILE), somdec=sd, **cx)
 
 
 def somdec_options(**kw) -> SolverOptions:
@@ -435,7 +444,8 @@ def somdec_options(**kw) -> SolverOptions:
 
 
 def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float = 1800.0,
-                    ld: Optional[int] = None, oxygen: Optional[float] = None) -> Workload:
+                    ld: Optional[int] = None, oxygen: Optional[float] = None,
+                    complexes: bool = False) -> Workload:
     """ELM-CN decomposition columns (500,000 columns x 15 layers by default):
     per-reaction first-order rates drawn U[1e-8, 1e-5] /s, SOM C:N (mass)
     U[8, 30], Monod half-saturation 1e-6 mol/m^3; per cell a temperature
@@ -444,12 +454,14 @@ def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float =
     (mass) U[20, 80], SOM C logU[1e-3, 1], NH4+ / NO3- logU[1e-7, 1e-4]
     mol/L, CO2 logU[1e-6, 1e-3] mol/L, N2O 1e-20 mol/L, pH 6; with oxygen
     (a Monod half saturation, mol/L) an O2(aq) primary logU[2e-4, 1e-3]
-    mol/L limits every reaction and is consumed as CO2 is released."""
+    mol/L limits every reaction and is consumed as CO2 is released; with
+    complexes the two ion pairs of somdec_network (free-molality guess =
+    totals, unit activity coefficients)."""
     rng = _rng(seed)
     rates = 10.0 ** rng.uniform(-8.0, -5.0, 7)
     som_cn = rng.uniform(8.0, 30.0, 4)
     net = somdec_network(rates=list(rates), som_cn=tuple(som_cn), half_saturation=1.0e-6,
-                         temperature_function=2, Q10=2.0, oxygen=oxygen)
+                         temperature_function=2, Q10=2.0, oxygen=oxygen, complexes=complexes)
     st = CellState(net, ncell, ld, somdec_scalars=True)
     n = ncell
     lu = lambda lo, hi, k: 10.0 ** rng.uniform(np.log10(lo), np.log10(hi), k)
@@ -459,6 +471,8 @@ def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float =
     st["total"][:, :n] = T
     st["pri_molal"][:, :n] = T          # no complexes, unit density below
     st["pri_act_coef"][:, :n] = 1.0
+    if complexes:
+        st["sec_act_coef"][:, :n] = 1.0
     litc = np.vstack([lu(1e-2, 1.0, n) for _ in range(3)])
     litcn = rng.uniform(20.0, 80.0, (3, n))
     from .somdec import CN_RATIO_MASS_TO_MOL
@@ -474,5 +488,6 @@ def somdec_workload(ncell: int = 500_000 * 15, seed: int = 20261205, dt: float =
     st["somdec_f_w"][0, :n] = rng.uniform(0.0, 1.0, n)
     notes = ("SomDec ELM-CN cascade (clm_cn3 structure); rates U[1e-8,1e-5]/s per reaction, "
              "SOM C:N U[8,30], Q10 = 2 with per-cell temperature, per-cell f_w U[0,1]" +
-             ("" if oxygen is None else ", Monod O2 response K = %g mol/L with O2 tracking" % oxygen))
+             ("" if oxygen is None else ", Monod O2 response K = %g mol/L with O2 tracking" % oxygen) +
+             (", NH4NO3(aq) / NH4CO2(aq) ion pairs" if complexes else ""))
     return Workload("somdec_%d" % ncell, net, somdec_options(), st, dt, seed, notes)

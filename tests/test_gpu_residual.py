@@ -34,6 +34,8 @@ def _case(which, ncell, seed=7):
         wl = workloads.somdec_workload(ncell=ncell)
     elif which == "somdec-oxygen":
         wl = workloads.somdec_workload(ncell=ncell, oxygen=1.0e-5)
+    elif which == "somdec-complexes":
+        wl = workloads.somdec_workload(ncell=ncell, complexes=True)
     elif which == "prefactor":
         import pin_util as PU
         wl = PU.prefactor_workload(ncell)
@@ -86,7 +88,8 @@ def _oracle(wl, st, fixed, fimm, cells, dt):
 
 
 @pytest.mark.parametrize("which,ncell", [("calcite", 1000), ("random15", 2048), ("somdec", 1024),
-                                         ("somdec-oxygen", 512), ("prefactor", 512)])
+                                         ("somdec-oxygen", 512), ("somdec-complexes", 512),
+                                         ("prefactor", 512)])
 def test_residual_jacobian_matches_oracle(which, ncell):
     import torch
     from pfchem.lib import Context, DeviceState

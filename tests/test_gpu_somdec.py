@@ -209,3 +209,27 @@ def test_somdec_oxygen_parity(oxygen_columns, variant):
     s1 = got["total"][ico2, :n] + got["total"][io2, :n]
     np.testing.assert_allclose(s1, s0, rtol=1e-12)
     assert np.all(got["total"][io2, :n] < wl.state["total"][io2, :n])
+
+
+@pytest.fixture(scope="module")
+def complex_columns():
+    wl = workloads.somdec_workload(4096, complexes=True)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    return wl, ref, rec, _oracle_band(wl, ref)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_somdec_with_complexes_parity(complex_columns, variant):
+    """the sandbox on a network with aqueous ion pairs (NH4NO3(aq),
+    NH4CO2(aq)): totals and the dtotal entries it reads include the
+    complexes; the oracle band as above, secondary molalities too"""
+    wl, ref, rec, band = complex_columns
+    got, _ = G.run_gpu(wl, variant)
+    _check(wl, ref, rec, band, got, "somdec-complexes/" + abi.KERNEL_NAMES[variant])
+    n = wl.state.ncell
+    a, b = ref["sec_molal"][:, :n], got["sec_molal"][:, :n]
+    same = ref["num_sub_steps"][0, :n] == got["num_sub_steps"][0, :n]
+    rel = np.abs(b - a)[:, same] / np.abs(a)[:, same]
+    print("somdec-complexes sec_molal max rel err", float(rel.max()), flush=True)
+    assert rel.max() < 1e-6

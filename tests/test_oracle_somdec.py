@@ -299,3 +299,30 @@ def test_oxygen_tracking_balances_co2():
         np.testing.assert_allclose(t1[0] + t1[3], t0[0] + t0[3], rtol=1e-13)
         np.testing.assert_allclose((t1[0] - t0[0]) * psv, (c0 - im[0]) * 1.0, rtol=1e-11)
         assert 0.0 < t1[3] < t0[3]
+
+
+def test_somdec_with_complexes_conserves_and_reduces_without_them():
+    """the sandbox on a network with aqueous complexes (NH4+ / NO3- / CO2
+    ion pairs) reads component totals and dtotal with the complexes' share:
+    C and N stay conserved over a step, and with the complexes' log K raised
+    until they vanish the step reproduces the network without them"""
+    import dataclasses
+    import gpu_util as G
+    n = 256
+    wl = workloads.somdec_workload(n, complexes=True)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    share = (ref["sec_molal"][0, :n] + ref["sec_molal"][1, :n]) / ref["total"][1, :n]
+    assert np.median(share) > 0.05
+    C0, N0 = SU.carbon_nitrogen(wl.net, wl.state.arrays, n)
+    C1, N1 = SU.carbon_nitrogen(wl.net, ref, n)
+    np.testing.assert_allclose(C1, C0, rtol=1e-12)
+    np.testing.assert_allclose(N1, N0, rtol=1e-11)
+    gone = dataclasses.replace(wl.net, eqcplx_logK=np.array([40.0, 40.0]))
+    a, _, nf_a = G.run_oracle(dataclasses.replace(wl, net=gone))
+    plain = workloads.somdec_workload(n)
+    b, _, nf_b = G.run_oracle(plain)
+    assert nf_a == 0 and nf_b == 0
+    for f in ("total", "immobile"):
+        np.testing.assert_allclose(a[f][:plain.net.naqcomp if f == "total" else None, :n],
+                                   b[f][:, :n], rtol=1e-12, atol=1e-300)
