@@ -93,7 +93,8 @@ def main():
     n = host.ncell
     log("[bench] workload %s: %d cells generated in %.1f s" % (wl.name, n, time.time() - t0))
     if args.only == "equilibrate":
-        print(json.dumps({"constraint_speciation": equilibrate_leg(net, opt, DeviceState(host))}),
+        print(json.dumps({"constraint_speciation": equilibrate_leg(
+            net, opt, DeviceState(host), host, args.cpu_threads or min(16, os.cpu_count() or 1))}),
               flush=True)
         return 0
 
@@ -211,7 +212,8 @@ def main():
     log("[bench] girt residual+jacobian %.2f ms  %.3e cells/s" % (g_ms, n / (g_ms * 1e-3)))
     del gst_, fixed
 
-    equil = equilibrate_leg(net, opt, copies(1)[0])
+    equil = equilibrate_leg(net, opt, copies(1)[0], host,
+                            args.cpu_threads or min(16, os.cpu_count() or 1))
 
     got = mix_state.to_host()
     status = got["status"][0, :n]
@@ -582,7 +584,7 @@ def run_configs(args):
     return res
 
 
-def equilibrate_leg(net, opt, cst):
+def equilibrate_leg(net, opt, cst, host=None, nthr=16):
     """constraint speciation of every cell of the device state cst from its
     own totals (pfchem_equilibrate, TOTAL constraints, fresh guess each
     call), with the reference's default constraint tolerances (1e-12
@@ -616,6 +618,28 @@ def equilibrate_leg(net, opt, cst):
              "cells_per_s": n / (c_ms * 1e-3),
              "mean_iterations": float(cgot["num_iterations"][0, :n].mean()),
              "failed_cells": int((cgot["status"][0, :n] != 0).sum())}
+    if host is not None:
+        # agreement with the oracle's ReactionEquilibrateConstraint on a
+        # strided sample of 2,048 cells (status, iterations, concentrations)
+        import numpy as np
+        import oracle_lib as O
+        cells = np.arange(0, n, max(1, n // 2048))[:2048]
+        sub = host.subset(cells)
+        ref = sub.copy()
+        O.equilibrate_batch(net, cctx.opt, con, ref, np.ascontiguousarray(sub["total"][:naq]), nthr)
+        g = cgot.subset(cells)
+        m = sub.ncell
+        ok = (ref["status"][0, :m] == 0) & (g["status"][0, :m] == 0)
+        err = 0.0
+        for f in ("pri_molal", "sec_molal"):
+            a, b = ref[f][:, :m][:, ok], g[f][:, :m][:, ok]
+            if a.size:
+                err = max(err, float(np.max(np.abs(b - a) / np.maximum(np.abs(a), 1e-300))))
+        equil["oracle_sample"] = {
+            "cells": int(m), "status_equal": float(np.mean(ref["status"][0, :m] == g["status"][0, :m])),
+            "iterations_equal": float(np.mean(ref["num_iterations"][0, :m] ==
+                                              g["num_iterations"][0, :m])),
+            "max_rel_conc_err": err}
     # algorithmic HBM bytes per cell: constraint values and density read;
     # free molalities, totals, activity coefficients, secondary molalities
     # and coefficients, iterations and status written
