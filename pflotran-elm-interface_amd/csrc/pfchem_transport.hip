@@ -15,7 +15,9 @@
 // with the row's role in the sign (id + 1: the row is the connection's
 // upstream cell, -(id + 1): downstream, 0: padding), so the values of a step
 // are formed row by row from the connection coefficients without atomics.
-// Vectors are [component][cell].
+// The solver's vectors hold the components in pairs, [pair][cell] of
+// double2 (component k is element k & 1 of pair k >> 1; an odd count has a
+// zero partner), so one 16-byte gather serves two components.
 //
 // Solver.  The Jacobi preconditioner is folded into the matrix as a column
 // scaling, A = M D^-1 (unit diagonal), and BiCGSTAB iterates y = D x on
@@ -299,15 +301,21 @@ __global__ __launch_bounds__(kTB) void k_assemble(
         }
       }
 #pragma unroll
-      for (int u = 0; u < kKC; ++u)
+      for (int u = 0; u < kKC; ++u) {
+        const int k = k0 + u;
+        const size_t jv = ((size_t)(k >> 1) * n + i) * 2 + (k & 1);  // pair layout
         if (on[u]) {
           const double res = b[u] - mx[u];
-          y[(size_t)(k0 + u) * n + i] = d * x0[u];
-          r[(size_t)(k0 + u) * n + i] = res;
+          y[jv] = d * x0[u];
+          r[jv] = res;
           pr[u] = b[u] * b[u];
           pr[kKC + u] = res * res;
-          pr[2 * kKC + u] = shadow(i, k0 + u) * res;
+          pr[2 * kKC + u] = shadow(i, k) * res;
+        } else if (k == naq && (naq & 1)) {
+          y[jv] = 0.0;  // the zero partner of an odd component count
+          r[jv] = 0.0;
         }
+      }
     }
     wave_sums<3 * kKC>(pr);
     chunk_to_lds<3>(lds, naq, k0, on, lane, pr);
@@ -323,7 +331,8 @@ __global__ __launch_bounds__(kTB) void k_scale(int64_t nent, const int32_t* __re
   if (e < nent) val[e] = val[e] / diag[col[e]];
 }
 
-// p = r (first iteration) or r + beta (p - omega v), active components
+// p = r (first iteration) or r + beta (p - omega v); values of inactive
+// components are computed with zero scalars and never read
 __global__ __launch_bounds__(kTB) void k_p(int64_t n, int naq, int first,
                                            const int32_t* __restrict__ act,
                                            const double* __restrict__ sc,
@@ -331,29 +340,38 @@ __global__ __launch_bounds__(kTB) void k_p(int64_t n, int naq, int first,
                                            const double* __restrict__ v) {
   const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (i >= n) return;
+  const int npair = (naq + 1) >> 1;
+  const double2* r2 = (const double2*)r;
+  const double2* v2 = (const double2*)v;
+  double2* p2 = (double2*)p;
   for (int k0 = 0; k0 < naq; k0 += kKC) {
     bool on[kKC];
     if (!chunk_on(act, naq, k0, on)) continue;  // uniform
-    double rj[kKC], pj[kKC], vj[kKC];
+    const int P0 = k0 >> 1;
+    const bool has1 = P0 + 1 < npair;
+    const size_t j0 = (size_t)P0 * n + i, j1 = (size_t)(has1 ? P0 + 1 : P0) * n + i;
+    const double2 ra = r2[j0], rb = r2[j1];
+    if (first) {
+      p2[j0] = ra;
+      if (has1) p2[j1] = rb;
+      continue;
+    }
+    const double2 pa = p2[j0], pb = p2[j1], va = v2[j0], vb = v2[j1];
+    double be[kKC], om[kKC];
 #pragma unroll
     for (int u = 0; u < kKC; ++u) {
-      const size_t j = (size_t)comp(on, k0, u) * n + i;
-      rj[u] = r[j];
-      pj[u] = first ? 0.0 : p[j];
-      vj[u] = first ? 0.0 : v[j];
+      be[u] = on[u] ? sc[(k0 + u) * kSc + kBeta] : 0.0;
+      om[u] = on[u] ? sc[(k0 + u) * kSc + kOmega] : 0.0;
     }
-#pragma unroll
-    for (int u = 0; u < kKC; ++u)
-      if (on[u]) {
-        const int k = k0 + u;
-        p[(size_t)k * n + i] =
-            first ? rj[u] : rj[u] + sc[k * kSc + kBeta] * (pj[u] - sc[k * kSc + kOmega] * vj[u]);
-      }
+    p2[j0] = make_double2(ra.x + be[0] * (pa.x - om[0] * va.x), ra.y + be[1] * (pa.y - om[1] * va.y));
+    if (has1)
+      p2[j1] = make_double2(rb.x + be[2] * (pb.x - om[2] * vb.x), rb.y + be[3] * (pb.y - om[3] * vb.y));
   }
 }
 
 // yv = A xv for the active components, with the block partials of
-// MODE 0: (rh, yv);  MODE 1: (yv, xv) and (yv, yv)
+// MODE 0: (rh, yv);  MODE 1: (yv, xv) and (yv, yv).  Each gather loads a
+// pair of components (16 bytes).
 template <int MODE, int KW>
 __global__ __launch_bounds__(kTB) void k_spmv(int64_t n, int naq, int64_t nslice,
                                               const int32_t* __restrict__ act,
@@ -370,6 +388,9 @@ __global__ __launch_bounds__(kTB) void k_spmv(int64_t n, int naq, int64_t nslice
   const bool ok = i < n;
   const int64_t e0 = sl < nslice ? sptr[sl] : 0;
   const int w = sl < nslice ? (int)((sptr[sl + 1] - e0) >> 6) : 0;
+  const int npair = (naq + 1) >> 1;
+  const double2* x2 = (const double2*)xv;
+  double2* y2 = (double2*)yv;
   double av[KW];
   int cv[KW];
   // the entry arrays hold kMaxKW x 64 entries past the last slice, so the
@@ -390,29 +411,36 @@ __global__ __launch_bounds__(kTB) void k_spmv(int64_t n, int naq, int64_t nslice
 #pragma unroll
     for (int u = 0; u < NQ * kKC; ++u) pr[u] = 0.0;
     if (ok) {
-      const double* xk[kKC];
-      double xi[kKC], a[kKC];
+      const int P0 = k0 >> 1;
+      const bool has1 = P0 + 1 < npair;
+      const double2* xa = x2 + (size_t)P0 * n;
+      const double2* xb = x2 + (size_t)(has1 ? P0 + 1 : P0) * n;
+      const double2 xia = xa[i], xib = xb[i];
+      double a[kKC] = {xia.x, xia.y, xib.x, xib.y};
+      const double xi[kKC] = {xia.x, xia.y, xib.x, xib.y};
 #pragma unroll
-      for (int u = 0; u < kKC; ++u) {
-        xk[u] = xv + (size_t)comp(on, k0, u) * n;
-        xi[u] = xk[u][i];
-        a[u] = xi[u];
+      for (int j = 0; j < KW; ++j) {
+        const double2 ga = xa[cv[j]], gb = xb[cv[j]];
+        a[0] += av[j] * ga.x;
+        a[1] += av[j] * ga.y;
+        a[2] += av[j] * gb.x;
+        a[3] += av[j] * gb.y;
       }
-#pragma unroll
-      for (int j = 0; j < KW; ++j)
-#pragma unroll
-        for (int u = 0; u < kKC; ++u) a[u] += av[j] * xk[u][cv[j]];
       for (int j = KW; j < w; ++j) {
         const int64_t e = e0 + (int64_t)j * 64 + lane;
         const double m = val[e];
         const int c = col[e];
-#pragma unroll
-        for (int u = 0; u < kKC; ++u) a[u] += m * xk[u][c];
+        const double2 ga = xa[c], gb = xb[c];
+        a[0] += m * ga.x;
+        a[1] += m * ga.y;
+        a[2] += m * gb.x;
+        a[3] += m * gb.y;
       }
+      y2[(size_t)P0 * n + i] = make_double2(a[0], a[1]);
+      if (has1) y2[(size_t)(P0 + 1) * n + i] = make_double2(a[2], a[3]);
 #pragma unroll
       for (int u = 0; u < kKC; ++u)
         if (on[u]) {
-          yv[(size_t)(k0 + u) * n + i] = a[u];
           if (MODE == 0) {
             pr[u] = shadow(i, k0 + u) * a[u];
           } else {
@@ -427,33 +455,34 @@ __global__ __launch_bounds__(kTB) void k_spmv(int64_t n, int naq, int64_t nslice
   block_partials(lds, part, NQ, naq, act, blk);
 }
 
-// s = r - alpha v, active components
+// s = r - alpha v (inactive components: values never read)
 __global__ __launch_bounds__(kTB) void k_s(int64_t n, int naq, const int32_t* __restrict__ act,
                                            const double* __restrict__ sc,
                                            const double* __restrict__ r,
                                            const double* __restrict__ v, double* __restrict__ s) {
   const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (i >= n) return;
+  const int npair = (naq + 1) >> 1;
+  const double2* r2 = (const double2*)r;
+  const double2* v2 = (const double2*)v;
+  double2* s2 = (double2*)s;
   for (int k0 = 0; k0 < naq; k0 += kKC) {
     bool on[kKC];
     if (!chunk_on(act, naq, k0, on)) continue;  // uniform
-    double rj[kKC], vj[kKC];
+    const int P0 = k0 >> 1;
+    const bool has1 = P0 + 1 < npair;
+    const size_t j0 = (size_t)P0 * n + i, j1 = (size_t)(has1 ? P0 + 1 : P0) * n + i;
+    const double2 ra = r2[j0], rb = r2[j1], va = v2[j0], vb = v2[j1];
+    double al[kKC];
 #pragma unroll
-    for (int u = 0; u < kKC; ++u) {
-      const size_t j = (size_t)comp(on, k0, u) * n + i;
-      rj[u] = r[j];
-      vj[u] = v[j];
-    }
-#pragma unroll
-    for (int u = 0; u < kKC; ++u)
-      if (on[u]) {
-        const int k = k0 + u;
-        s[(size_t)k * n + i] = rj[u] - sc[k * kSc + kAlpha] * vj[u];
-      }
+    for (int u = 0; u < kKC; ++u) al[u] = on[u] ? sc[(k0 + u) * kSc + kAlpha] : 0.0;
+    s2[j0] = make_double2(ra.x - al[0] * va.x, ra.y - al[1] * va.y);
+    if (has1) s2[j1] = make_double2(rb.x - al[2] * vb.x, rb.y - al[3] * vb.y);
   }
 }
 
-// y += alpha p + omega s; r = s - omega t; block partials of (r, r), (rh, r)
+// y += alpha p + omega s (active components; the others keep their
+// iterate); r = s - omega t; block partials of (r, r), (rh, r)
 __global__ __launch_bounds__(kTB) void k_x(int64_t n, int naq,
                                            const int32_t* __restrict__ act,
                                            const double* __restrict__ sc,
@@ -465,6 +494,12 @@ __global__ __launch_bounds__(kTB) void k_x(int64_t n, int naq,
   const int64_t i = xcd_block() * kTB + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool ok = i < n;
+  const int npair = (naq + 1) >> 1;
+  const double2* p2 = (const double2*)p;
+  const double2* s2 = (const double2*)s;
+  const double2* t2 = (const double2*)t;
+  double2* y2 = (double2*)y;
+  double2* r2 = (double2*)r;
   for (int k0 = 0; k0 < naq; k0 += kKC) {
     bool on[kKC];
     if (!chunk_on(act, naq, k0, on)) continue;  // uniform
@@ -472,27 +507,32 @@ __global__ __launch_bounds__(kTB) void k_x(int64_t n, int naq,
 #pragma unroll
     for (int u = 0; u < 2 * kKC; ++u) pr[u] = 0.0;
     if (ok) {
-      double pj[kKC], sj[kKC], tj[kKC], yj[kKC];
+      const int P0 = k0 >> 1;
+      const bool has1 = P0 + 1 < npair;
+      const size_t j0 = (size_t)P0 * n + i, j1 = (size_t)(has1 ? P0 + 1 : P0) * n + i;
+      const double2 pa = p2[j0], pb = p2[j1], sa = s2[j0], sb = s2[j1];
+      const double2 ta = t2[j0], tb = t2[j1], ya = y2[j0], yb = y2[j1];
+      const double pj[kKC] = {pa.x, pa.y, pb.x, pb.y}, sj[kKC] = {sa.x, sa.y, sb.x, sb.y};
+      const double tj[kKC] = {ta.x, ta.y, tb.x, tb.y};
+      double yj[kKC] = {ya.x, ya.y, yb.x, yb.y}, rj[kKC];
 #pragma unroll
       for (int u = 0; u < kKC; ++u) {
-        const size_t j = (size_t)comp(on, k0, u) * n + i;
-        pj[u] = p[j];
-        sj[u] = s[j];
-        tj[u] = t[j];
-        yj[u] = y[j];
-      }
-#pragma unroll
-      for (int u = 0; u < kKC; ++u)
+        const int k = k0 + u;
+        const double al = on[u] ? sc[k * kSc + kAlpha] : 0.0;
+        const double om = on[u] ? sc[k * kSc + kOmega] : 0.0;
+        if (on[u]) yj[u] = yj[u] + (al * pj[u] + om * sj[u]);
+        rj[u] = sj[u] - om * tj[u];
         if (on[u]) {
-          const int k = k0 + u;
-          const size_t j = (size_t)k * n + i;
-          const double al = sc[k * kSc + kAlpha], om = sc[k * kSc + kOmega];
-          y[j] = yj[u] + (al * pj[u] + om * sj[u]);
-          const double rr = sj[u] - om * tj[u];
-          r[j] = rr;
-          pr[u] = rr * rr;
-          pr[kKC + u] = shadow(i, k) * rr;
+          pr[u] = rj[u] * rj[u];
+          pr[kKC + u] = shadow(i, k) * rj[u];
         }
+      }
+      y2[j0] = make_double2(yj[0], yj[1]);
+      r2[j0] = make_double2(rj[0], rj[1]);
+      if (has1) {
+        y2[j1] = make_double2(yj[2], yj[3]);
+        r2[j1] = make_double2(rj[2], rj[3]);
+      }
     }
     wave_sums<2 * kKC>(pr);
     chunk_to_lds<2>(lds, naq, k0, on, lane, pr);
@@ -573,15 +613,11 @@ __global__ __launch_bounds__(kTB) void k_store(int64_t n, int naq, const double*
   const int64_t i = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (i >= n) return;
   const double id = 1.0 / diag[i];
-  for (int k0 = 0; k0 < naq; k0 += kKC) {
-    bool on[kKC];
-    chunk_on(nullptr, naq, k0, on);
-    double yj[kKC];
-#pragma unroll
-    for (int u = 0; u < kKC; ++u) yj[u] = y[(size_t)comp(on, k0, u) * n + i];
-#pragma unroll
-    for (int u = 0; u < kKC; ++u)
-      if (on[u]) total[(size_t)(k0 + u) * ld + i] = yj[u] * id;
+  const double2* y2 = (const double2*)y;
+  for (int P = 0; 2 * P < naq; ++P) {
+    const double2 v = y2[(size_t)P * n + i];
+    total[(size_t)(2 * P) * ld + i] = v.x * id;
+    if (2 * P + 1 < naq) total[(size_t)(2 * P + 1) * ld + i] = v.y * id;
   }
 }
 
@@ -741,7 +777,7 @@ extern "C" int pfchem_transport_step(pfchem_ctx* ctx, int64_t ncell, const pfche
   // partials (3 products), scalars, flags
   const int64_t n = ncell, ns = p->nslice;
   const int64_t nblk = (ns + 3) / 4;  // 4 slices per block
-  const size_t vec = (size_t)naq * n;
+  const size_t vec = (size_t)((naq + 1) & ~1) * n;  // pairs of components
   const size_t need = ((size_t)n + (size_t)(p->nent + kEntryPad) + 6 * vec +
                        (size_t)3 * naq * nblk + (size_t)kSc * naq) * sizeof(double) +
                       2 * (size_t)naq * sizeof(int32_t) + 256;
