@@ -23,7 +23,10 @@ struct pfchem_ctx {
   int num_cus = 256;
   DevNet dnet{};
   void* dbuf = nullptr;  // device copy of the network arrays
-  unsigned long long* counter = nullptr;  // per-XCD work cursors (128-byte stride)
+  // per-XCD work cursors (128-byte stride), one block of kCursorWords per
+  // descriptor slot: a launch uses its slot's block, so launches on
+  // different streams never share cursors
+  unsigned long long* counter = nullptr;
   int nxcd = 8;                           // XCDs of the device
   int max_lds = 65536;                    // LDS bytes per workgroup
   DevNet* dnet_dev = nullptr;             // device copy of dnet
@@ -37,6 +40,13 @@ struct pfchem_ctx {
   LaunchCfg vlaunch[2][kBuckets];
   int32_t* buckets = nullptr;  // kBuckets lists of ncell entries, then the counts
   int64_t bucket_cap = 0;
+  int32_t* esc_list = nullptr;  // cells the one-lane pass handed over, then their count
+  int64_t esc_cap = 0;
+  // steps that use the scratch lists held here (escalated cells, status
+  // scratch, size buckets) are ordered on the device: each records this
+  // event after its last launch and the next one's stream waits for it
+  hipEvent_t scratch_done = nullptr;
+  bool scratch_done_used = false;
   pfchem_network net{};   // sizes only (host pointers are not kept)
   pfchem_options opt{};
   bool has_somdec = false;
@@ -131,6 +141,25 @@ void* static_scratch_ptr(int device) {
 // host mirror) is rewritten, that event must have completed, whatever
 // stream the earlier launch used.
 const int kStateRing = 256;
+constexpr size_t kCursorWords = 1024;  // 8 KiB of cursors per descriptor slot
+
+unsigned long long* slot_cursors(pfchem_ctx* ctx, int k) {
+  return ctx->counter + (size_t)k * kCursorWords;
+}
+
+// before a scratch list is freed / regrown: every step that used it is done
+void scratch_quiesce(pfchem_ctx* ctx) {
+  if (ctx->scratch_done_used) (void)hipEventSynchronize(ctx->scratch_done);
+}
+// a step that uses the scratch lists starts after the previous such step
+hipError_t scratch_order(pfchem_ctx* ctx, hipStream_t s) {
+  return ctx->scratch_done_used ? hipStreamWaitEvent(s, ctx->scratch_done, 0) : hipSuccess;
+}
+hipError_t scratch_release(pfchem_ctx* ctx, hipStream_t s) {
+  hipError_t e = hipEventRecord(ctx->scratch_done, s);
+  if (e == hipSuccess) ctx->scratch_done_used = true;
+  return e;
+}
 
 DevState* acquire_slot(pfchem_ctx* ctx, const DevState& d, hipStream_t s, int* idx,
                        hipError_t* e) {
@@ -517,7 +546,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     return PFCHEM_ERR_NOMEM;
   }
   e = hipMemcpy(ctx->dbuf, P.bytes.data(), P.bytes.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&ctx->counter, 128 * 64);
+  if (e == hipSuccess) e = hipMalloc(&ctx->counter, sizeof(unsigned long long) * kCursorWords * kStateRing);
   if (e != hipSuccess) {
     (void)hipFree(ctx->dbuf);
     if (ctx->counter) (void)hipFree(ctx->counter);
@@ -703,6 +732,10 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
       pfchem_destroy(ctx);
       return PFCHEM_ERR_HIP;
     }
+  if (hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming) != hipSuccess) {
+    pfchem_destroy(ctx);
+    return PFCHEM_ERR_HIP;
+  }
   // the FP64 team shape also takes the cells the lane kernel hands over
   ctx->team_default = ctx->launch[PFCHEM_KERNEL_BATCHED_FP64];
   // network-specialized one-cell-per-lane kernel: the default batched
@@ -784,6 +817,7 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->counter) (void)hipFree(ctx->counter);
   if (ctx->stage) (void)hipFree(ctx->stage);
   if (ctx->buckets) (void)hipFree(ctx->buckets);
+  if (ctx->esc_list) (void)hipFree(ctx->esc_list);
   if (ctx->dnet_dev) (void)hipFree(ctx->dnet_dev);
   if (ctx->dstate_ring) (void)hipFree(ctx->dstate_ring);
   if (ctx->hstate_ring) (void)hipHostFree(ctx->hstate_ring);
@@ -798,6 +832,7 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   if (ctx->tplan) transport_plan_free(ctx->tplan);
   for (int k = 0; k < kStateRing; ++k)
     if (ctx->ring_done[k]) (void)hipEventDestroy(ctx->ring_done[k]);
+  if (ctx->scratch_done) (void)hipEventDestroy(ctx->scratch_done);
   delete ctx;
   return PFCHEM_OK;
 }
@@ -861,6 +896,7 @@ static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double d
   int32_t* st = d.status;
   if (!st) {
     if (ncell > ctx->status_cap) {
+      scratch_quiesce(ctx);
       if (ctx->status_scratch) (void)hipFree(ctx->status_scratch);
       ctx->status_scratch = nullptr;
       ctx->status_cap = 0;
@@ -896,20 +932,45 @@ static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double d
   a.escalate = 1;
   const int bpc = ctx->launch[mixed ? PFCHEM_KERNEL_BATCHED_MIXED : PFCHEM_KERNEL_BATCHED_FP64]
                       .blocks_per_cu;
-  hipError_t e;
+  hipError_t e = hipSuccess;
   a.rescue = 0;
-  e = lane_launch(ctx->lane_src, ctx->device, mixed ? 1 : 0, a, ctx->num_cus, bpc, ctx->counter,
-                  s);
-  if (e != hipSuccess)
-    return hip_fail(ctx, e, mixed ? "lane kernel launch (FP32)" : "lane kernel launch (FP64)");
+  // the rescue pass walks only the cells the one-lane pass handed over
+  // (listed by a compaction kernel; larger grids walk all cells)
   DevState dr = d;
   dr.status = st;
+  const bool listed = ncell <= 0x7fffffff;
+  if (listed) {
+    if (ncell > ctx->esc_cap) {
+      scratch_quiesce(ctx);
+      if (ctx->esc_list) (void)hipFree(ctx->esc_list);
+      ctx->esc_list = nullptr;
+      ctx->esc_cap = 0;
+      if (hipMalloc(&ctx->esc_list, sizeof(int32_t) * ((size_t)ncell + 64)) != hipSuccess)
+        return fail(ctx, PFCHEM_ERR_NOMEM, "escalated-cell list");
+      ctx->esc_cap = ncell;
+    }
+    dr.cell_list = ctx->esc_list;
+    dr.list_count = ctx->esc_list + ncell;
+  }
+  // one descriptor slot (and its cursors) for both launches of this step
   int k = 0;
   DevState* slot = acquire_slot(ctx, dr, s, &k, &e);
   if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
+  unsigned long long* cursors = slot_cursors(ctx, k);
+  // a step on another stream may still use the list / status scratch
+  e = scratch_order(ctx, s);
+  if (e == hipSuccess)
+    e = lane_launch(ctx->lane_src, ctx->device, mixed ? 1 : 0, a, ctx->num_cus, bpc, cursors, s);
+  if (e != hipSuccess)
+    return hip_fail(ctx, e, mixed ? "lane kernel launch (FP32)" : "lane kernel launch (FP64)");
+  if (listed) {
+    e = launch_escalated_list(st, ncell, ctx->esc_list, ctx->esc_list + ncell, s);
+    if (e != hipSuccess) return hip_fail(ctx, e, "escalated-cell list");
+  }
   e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, kTeamRescue64,
-                       ctx->team_default, ctx->counter, ctx->num_cus, ctx->nxcd, s, true);
+                       ctx->team_default, cursors, ctx->num_cus, ctx->nxcd, s, true);
   if (e == hipSuccess) e = release_slot(ctx, k, s);
+  if (e == hipSuccess) e = scratch_release(ctx, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "team rescue launch");
   return PFCHEM_OK;
 }
@@ -950,6 +1011,7 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
       // device, one launch per bucket with the bucket's team shape
       if (ncell > 0x7fffffff) return fail(ctx, PFCHEM_ERR_UNSUPPORTED, "variable_ncomp: ncell >= 2^31");
       if (ncell > ctx->bucket_cap) {
+        scratch_quiesce(ctx);
         if (ctx->buckets) (void)hipFree(ctx->buckets);
         ctx->buckets = nullptr;
         ctx->bucket_cap = 0;
@@ -958,7 +1020,9 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
         ctx->bucket_cap = ncell;
       }
       int32_t* counts = ctx->buckets + (size_t)kBuckets * ncell;
-      e = launch_buckets(d.ncomp_active, ncell, ctx->dnet.naq, ctx->buckets, counts, s);
+      e = scratch_order(ctx, s);
+      if (e == hipSuccess)
+        e = launch_buckets(d.ncomp_active, ncell, ctx->dnet.naq, ctx->buckets, counts, s);
       if (e != hipSuccess) return hip_fail(ctx, e, "bucket launch");
       for (int b = 0; b < kBuckets; ++b) {
         if (b > 0 && kBucketBound[b - 1] >= ctx->dnet.naq) break;  // no cell that large
@@ -971,12 +1035,14 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
         int k = 0;
         DevState* slot = acquire_slot(ctx, db, s, &k, &e);
         if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
-        e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cb, ctx->counter,
+        e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cb, slot_cursors(ctx, k),
                              ctx->num_cus, ctx->nxcd, s, d.status != nullptr);
         if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
         e = release_slot(ctx, k, s);
         if (e != hipSuccess) return hip_fail(ctx, e, "descriptor event");
       }
+      e = scratch_release(ctx, s);
+      if (e != hipSuccess) return hip_fail(ctx, e, "scratch event");
       return PFCHEM_OK;
     }
     const LaunchCfg& cfg = ctx->launch[variant];
@@ -986,7 +1052,7 @@ int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state, doubl
     int k = 0;
     DevState* slot = acquire_slot(ctx, d, s, &k, &e);
     if (e != hipSuccess) return hip_fail(ctx, e, "state descriptor copy");
-    e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cfg, ctx->counter,
+    e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, mixed, cfg, slot_cursors(ctx, k),
                          ctx->num_cus, ctx->nxcd, s, d.status != nullptr);
     if (e == hipSuccess) e = release_slot(ctx, k, s);
   }

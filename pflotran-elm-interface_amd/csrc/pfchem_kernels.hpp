@@ -90,6 +90,10 @@ hipError_t launch_team_step(const DevNet& n, const DevNet* dn, const DevState* d
 // counts are cleared by a fill kernel on `stream` first.
 constexpr int kBuckets = 3;
 constexpr int kBucketBound[kBuckets] = {16, 32, 64};
+// the cells whose status is kTeamEscalate into list[0 .. *count) (any
+// order); ncell < 2^31
+hipError_t launch_escalated_list(const int32_t* status, int64_t ncell, int32_t* list,
+                                 int32_t* count, hipStream_t stream);
 hipError_t launch_buckets(const int32_t* nact, int64_t ncell, int naq, int32_t* lists,
                           int32_t* counts, hipStream_t stream);
 

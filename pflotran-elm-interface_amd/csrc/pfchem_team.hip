@@ -96,7 +96,26 @@ bucket_cells(const int32_t* nact, int64_t ncell, int naq, int32_t* lists, int32_
   const int pos = atomicAdd(&counts[b], 1);
   lists[(int64_t)b * ncell + pos] = (int32_t)c;
 }
+__global__ void clear_count(int32_t* count) {
+  if (threadIdx.x < 1) count[threadIdx.x] = 0;
+}
+__global__ void __launch_bounds__(256)
+escalated_cells(const int32_t* status, int64_t ncell, int32_t* list, int32_t* count) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell || status[c] != kTeamEscalate) return;
+  list[atomicAdd(count, 1)] = (int32_t)c;
+}
 }  // namespace
+
+hipError_t launch_escalated_list(const int32_t* status, int64_t ncell, int32_t* list,
+                                 int32_t* count, hipStream_t stream) {
+  clear_count<<<1, 64, 0, stream>>>(count);
+  const int block = 256;
+  if (ncell > 0)
+    escalated_cells<<<dim3((unsigned)((ncell + block - 1) / block)), dim3(block), 0, stream>>>(
+        status, ncell, list, count);
+  return hipGetLastError();
+}
 
 hipError_t launch_buckets(const int32_t* nact, int64_t ncell, int naq, int32_t* lists,
                           int32_t* counts, hipStream_t stream) {

@@ -565,8 +565,9 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 
   // XCD of this workgroup (workgroups are dispatched round robin over XCDs)
   const int xcd = (int)(blockIdx.x % (unsigned)nxcd);
-  // cells of this launch: all, or with VAR a size bucket's list
-  const bool listed = VAR && s.cell_list != nullptr;
+  // cells of this launch: all, or a list (a size bucket's, or the cells an
+  // earlier pass handed over)
+  const bool listed = s.cell_list != nullptr;
   const int64_t nwork = listed ? (int64_t)gld(s.list_count) : ncell;
   const int64_t chunk = (nwork + nxcd - 1) / nxcd;
 
