@@ -594,7 +594,14 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   // wave-local queue of claimed cells [qnext, qend) (wave-uniform)
   long long qnext = 0, qend = 0;
   int steal = 0;  // XCD ranges found empty so far (wave-uniform)
-  const int batch = ((64 / L) > kClaimAlign ? (64 / L) : kClaimAlign);
+  int batch = ((64 / L) > kClaimAlign ? (64 / L) : kClaimAlign);
+  if (listed) {
+    // list entries are scattered cells (no lines to keep whole): a short
+    // list is claimed one cell per team, so that every team gets a cell
+    // rather than a few waves running the whole list several cells deep
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    if (nwork < 4 * (int64_t)batch * waves) batch = 64 / L;
+  }
 
   for (;;) {
     // ---------------- work stealing: hand out cells ----------------
