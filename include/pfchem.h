@@ -193,6 +193,13 @@ typedef struct pfchem_options {
                                             each cell's active primaries (see
                                             there); 0: every cell solves the
                                             whole network                   */
+  int32_t maximum_rreact_calls;          /* RReact calls one RStep may make
+                                            before the cell fails with
+                                            status 1 (a bound on the time a
+                                            stuck cell holds its lanes; the
+                                            reference's loop has none);
+                                            0 = 10000                       */
+  int32_t pad_options;
   double max_dlnC_rreact;                /* default 5                       */
   double max_residual_tolerance;         /* default 1e-12                   */
   double max_rel_residual_tolerance;     /* default 1e-8                    */

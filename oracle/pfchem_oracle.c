@@ -17,7 +17,8 @@
  *  - SomDec stoichiometry that the reference stores in the shared sandbox
  *    object per cell visit (reaction_sandbox_somdec.F90:1817-1858) is kept
  *    per cell, starting from the Setup values.
- *  - RStep gives up after ORC_MAX_RREACT_ATTEMPTS RReact calls; the
+ *  - RStep gives up after options.maximum_rreact_calls RReact calls
+ *    (0: ORC_MAX_RREACT_ATTEMPTS); the
  *    reference's sub-step loop has no bound (its cut counter can go
  *    negative after runs of good sub-steps, reaction.F90:3721-3725).
  *  - Mineral rate prefactors (reaction_mineral.F90:806-870) and sorption,
@@ -812,7 +813,9 @@ int orc_rstep(const pfchem_network* net, const pfchem_options* opt,
   if (orc_log) orc_log->n = orc_log->overflow = 0;
   for (;;) {
     if (cumulative_time >= target_time) break;
-    if (++attempts > ORC_MAX_RREACT_ATTEMPTS) return last_reason ? last_reason : 1;
+    if (++attempts > (opt->maximum_rreact_calls > 0 ? opt->maximum_rreact_calls
+                                                    : ORC_MAX_RREACT_ATTEMPTS))
+      return last_reason ? last_reason : 1;
     int inner = 0, reason = 0;
     if (orc_log)
       for (int i = 0; i < naq; ++i) t_start[i] = c->total[i];

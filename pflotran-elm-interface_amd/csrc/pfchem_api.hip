@@ -388,7 +388,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
     return PFCHEM_ERR_UNSUPPORTED;
   }
   if (opt->max_dlnC_rreact <= 0.0 || opt->maximum_reaction_iterations < 1 ||
-      opt->maximum_reaction_cuts < 0) {
+      opt->maximum_reaction_cuts < 0 || opt->maximum_rreact_calls < 0) {
     delete ctx;
     return PFCHEM_ERR_INVALID;
   }
@@ -574,6 +574,7 @@ int pfchem_create(const pfchem_network* net, const pfchem_options* opt, int devi
   d.isothermal = opt->isothermal;
   d.max_it = opt->maximum_reaction_iterations;
   d.max_cuts = opt->maximum_reaction_cuts;
+  d.max_calls = opt->maximum_rreact_calls > 0 ? opt->maximum_rreact_calls : kMaxRReactAttempts;
   d.full_geochem = opt->use_full_geochemistry;
   d.max_dlnC = opt->max_dlnC_rreact;
   d.tol_abs = opt->max_residual_tolerance;

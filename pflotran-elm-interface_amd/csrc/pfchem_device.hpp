@@ -14,9 +14,10 @@ constexpr double kLogToLn = 2.30258509299;
 constexpr double kIdealGas = 8.31446;
 constexpr double kRtMinSaturation = 1.0e-40;
 
-// RReact calls allowed per RStep.  The reference's sub-step loop has no
-// bound (its cut counter can go negative after runs of good sub-steps,
-// reaction.F90:3721-3725); every kernel loop here must end.
+// RReact calls allowed per RStep when pfchem_options.maximum_rreact_calls
+// is 0.  The reference's sub-step loop has no bound (its cut counter can go
+// negative after runs of good sub-steps, reaction.F90:3721-3725); every
+// kernel loop here must end.
 constexpr int kMaxRReactAttempts = 10000;
 
 enum FailReason : int { kOk = 0, kMaxIter = 1, kSingular = 2, kActivity = 3 };
@@ -75,6 +76,7 @@ struct DevNet {
   // option flags
   int log_form, act_freq, act_alg, act_h2o, total_as_guess, isothermal;
   int max_it, max_cuts, full_geochem;
+  int max_calls;  // RReact calls per RStep before the cell fails
   double max_dlnC, tol_abs, tol_rel, tol_change;
   double A, B, Bdot;
   // optional-array presence

@@ -188,6 +188,7 @@ std::string lane_source(const pfchem_network* net, const pfchem_options* opt) {
   g.i("ISOTHERMAL", opt->isothermal ? 1 : 0);
   g.i("MAX_IT", opt->maximum_reaction_iterations);
   g.i("MAX_CUTS", opt->maximum_reaction_cuts);
+  g.i("MAX_CALLS", opt->maximum_rreact_calls > 0 ? opt->maximum_rreact_calls : 10000);
   g.d("MAX_DLNC", opt->max_dlnC_rreact);
   g.d("TOL_ABS", opt->max_residual_tolerance);
   g.d("TOL_REL", opt->max_rel_residual_tolerance);

@@ -44,7 +44,6 @@ typedef long long i64;
 constexpr double kLogToLn = 2.30258509299;
 constexpr double kIdealGas = 8.31446;
 constexpr double kRtMinSat = 1.0e-40;
-constexpr int kMaxAttempts = 10000;  // RReact calls per RStep (the loop must end)
 constexpr int kEscalate = 16;        // per-cell status: handed to the next pass
 constexpr int kClaim = 64;           // cells per refill of a wave's queue
 enum : int { kOk = 0, kMaxIter = 1, kSingular = 2 };
@@ -394,7 +393,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
 
     // ---------------- RReact set-up (reaction.F90:3800-3851) ----------------
     if (mode == 1 && nit < 0) {
-      if (++attempts > kMaxAttempts) {
+      if (++attempts > Net::MAX_CALLS) {
         status = last_why ? last_why : kMaxIter;
         mode = 2;
       } else {
@@ -1184,7 +1183,7 @@ __device__ inline void lane_adjoint(const SensArgs& A) {
     double cum = 0.0, tdt = A.dt;
     bool handover = false;
     while (cum < A.dt) {
-      if (++attempts > kMaxAttempts) {
+      if (++attempts > Net::MAX_CALLS) {
         st = kMaxIter;
         break;
       }

@@ -548,7 +548,7 @@ plain_step_kernel(const DevNet* np, DevState s, int64_t ncell, double dt) {
     int nconst = 0, ncuts = 0, attempts = 0, last_why = 0;
     for (;;) {
       if (cum >= target) break;
-      if (++attempts > kMaxRReactAttempts) {
+      if (++attempts > n.max_calls) {
         status = last_why ? last_why : kMaxIter;
         break;
       }

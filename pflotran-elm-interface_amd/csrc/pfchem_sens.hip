@@ -273,7 +273,7 @@ sens_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
   double target = dt, cum = 0.0, tdt = dt;
   int nconst = 0, ncuts = 0, attempts = 0, st = 0;
   while (cum < target) {
-    if (++attempts > kMaxRReactAttempts) {
+    if (++attempts > n.max_calls) {
       st = PFCHEM_CELL_MAX_CUTS_ITERATIONS;
       break;
     }

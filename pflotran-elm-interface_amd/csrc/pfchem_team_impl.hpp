@@ -747,8 +747,8 @@ team_step_kernel(const DevNet* np, const DevState* sp, int64_t ncell, double dt,
 
     // ---------------- RReact set-up (reaction.F90:3800-3851) ----------------
     if (nit < 0) {
-      if (++attempts > kMaxRReactAttempts) {
-        // safety bound on RReact calls per RStep: fail the cell
+      if (++attempts > n.max_calls) {
+        // bound on RReact calls per RStep: fail the cell
         status = last_why ? last_why : kMaxIter;
 #pragma unroll
         for (int r = 0; r < R; ++r) {

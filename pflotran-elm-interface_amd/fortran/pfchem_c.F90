@@ -110,6 +110,8 @@ module pfchem_c
     integer(c_int32_t) :: maximum_reaction_iterations
     integer(c_int32_t) :: maximum_reaction_cuts
     integer(c_int32_t) :: variable_ncomp
+    integer(c_int32_t) :: maximum_rreact_calls = 0
+    integer(c_int32_t) :: pad_options = 0
     real(c_double) :: max_dlnC_rreact
     real(c_double) :: max_residual_tolerance
     real(c_double) :: max_rel_residual_tolerance

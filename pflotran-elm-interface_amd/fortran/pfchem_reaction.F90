@@ -382,6 +382,7 @@ contains
     opt%maximum_reaction_iterations = reaction%maximum_reaction_iterations
     opt%maximum_reaction_cuts = reaction%maximum_reaction_cuts
     opt%variable_ncomp = 0
+    opt%maximum_rreact_calls = 0
     opt%max_dlnC_rreact = reaction%max_dlnC_rreact
     opt%max_residual_tolerance = reaction%max_residual_tolerance
     opt%max_rel_residual_tolerance = reaction%max_rel_residual_tolerance

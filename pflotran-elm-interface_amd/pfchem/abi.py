@@ -187,6 +187,8 @@ class Options(C.Structure):
         ("maximum_reaction_iterations", C.c_int32),
         ("maximum_reaction_cuts", C.c_int32),
         ("variable_ncomp", C.c_int32),
+        ("maximum_rreact_calls", C.c_int32),
+        ("pad_options", C.c_int32),
         ("max_dlnC_rreact", C.c_double),
         ("max_residual_tolerance", C.c_double),
         ("max_rel_residual_tolerance", C.c_double),

@@ -318,6 +318,9 @@ class SolverOptions:
     max_relative_change_tolerance: float = 1.0e-6
     # per-cell active primary counts in the state (pfchem_state.ncomp_active)
     variable_ncomp: bool = False
+    # RReact calls per RStep before the cell fails (0: 10000); the
+    # reference's sub-step loop has no bound (reaction.F90:3721-3725)
+    maximum_rreact_calls: int = 0
 
     def to_abi(self) -> abi.Options:
         o = abi.Options()

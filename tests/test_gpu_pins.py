@@ -6,7 +6,8 @@ kinetic-mineral rate options (reaction_mineral.F90:729-908) and rate
 prefactors (:820-870, 964-1033; the one-lane kernel does not take them, so
 the batched variants run the team kernel), and
 RActivityCoefficients LAG through pfchem_update_activity
-(reaction.F90:4555-4611).  Every kernel variant and both batched kernels (the
+(reaction.F90:4555-4611), and the bound on RReact calls per RStep
+(options.maximum_rreact_calls).  Every kernel variant and both batched kernels (the
 one-lane kernel, the default, and a 16x1 team) run through the C ABI against
 the FP64 oracle on the same seeded cells.
 
@@ -151,3 +152,51 @@ def test_step_newton_activity_per_iteration():
     with pytest.raises(PfchemError) as e:
         ctx.step(DeviceState(wl.state), wl.dt, abi.KERNEL_BATCHED_FP64)
     assert e.value.code == abi.PFCHEM_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("variant,lanes", KERNELS, ids=IDS)
+def test_rreact_call_bound(variant, lanes):
+    """options.maximum_rreact_calls: with 5 Newton iterations allowed many
+    cells cut dt, and a bound of 3 RReact calls fails the cells that need
+    more.  The bound only truncates the sub-step loop, so against the same
+    kernel's unbounded run every cell fails or ends equal to it, cells that
+    took more than 3 sub-steps fail and one-sub-step cells do not (the mixed
+    variants hand a failing cell to the FP64 pass, which starts it again
+    with its own count).  Against the oracle under the same bound: the same
+    failing cells (a cell whose fifth Newton iteration sits at the
+    convergence test can fall either way under device rounding, so 1% may
+    differ) and the oracle's concentrations where both end with the same
+    sub-steps."""
+    import dataclasses
+    wl = workloads.random_network_workload(ncell=512)
+    opt = dataclasses.replace(wl.opt, maximum_reaction_iterations=5)
+    free_wl = dataclasses.replace(wl, opt=opt)
+    wl = dataclasses.replace(wl, opt=dataclasses.replace(opt, maximum_rreact_calls=3))
+    n = wl.state.ncell
+    label = "call-bound/" + IDS[KERNELS.index((variant, lanes))]
+    print("case", label, flush=True)
+    free, _ = G.run_gpu(free_wl, variant, lanes=lanes)
+    got, _ = G.run_gpu(wl, variant, lanes=lanes)
+    f_ok = free["status"][0, :n] == 0
+    g_ok = got["status"][0, :n] == 0
+    f_sub = free["num_sub_steps"][0, :n]
+    assert np.all(f_ok[g_ok]), label
+    assert not np.any(g_ok[f_ok & (f_sub > 3)]), label
+    assert np.all(g_ok[f_ok & (f_sub == 1)]), label
+    assert (f_ok & (f_sub > 3)).sum() > 10 and (f_ok & (f_sub == 1)).sum() > 50, label
+    np.testing.assert_array_equal(got["num_sub_steps"][0, g_ok], f_sub[g_ok])
+    for f in ("pri_molal", "total", "sec_molal", "mnrl_volfrac"):
+        a, b = free[f][:, :n][:, g_ok], got[f][:, :n][:, g_ok]
+        assert np.max(np.abs(b - a) / np.maximum(np.abs(a), 1e-300)) <= 1e-12, (label, f)
+    ref, _, _ = G.run_oracle(wl)
+    r_ok = ref["status"][0, :n] == 0
+    agree = float(np.mean(r_ok == g_ok))
+    print(label, "status agreement with the oracle", agree, flush=True)
+    assert agree >= 0.99, (label, agree)
+    both = r_ok & g_ok & (ref["num_sub_steps"][0, :n] == got["num_sub_steps"][0, :n])
+    assert both.sum() >= 0.95 * r_ok.sum()
+    for f in ("pri_molal", "total", "sec_molal", "mnrl_volfrac"):
+        a, b = ref[f][:, :n][:, both], got[f][:, :n][:, both]
+        e = float(np.max(np.abs(b - a) / np.maximum(np.abs(a), 1e-300)))
+        print(label, f, e, flush=True)
+        assert e <= CONC_TOL, (label, f, e)

@@ -91,6 +91,28 @@ def test_hand_over_without_later_pass(calcite_host):
         np.testing.assert_allclose(got[f], ref2[f], rtol=1e-12, atol=0)
 
 
+def test_rreact_call_bound(tmp_path):
+    """the generated source stops RStep after options.maximum_rreact_calls
+    RReact calls as the oracle does: the same cells fail, the others end as
+    the oracle's (the FP64 pass; no later pass)"""
+    import dataclasses
+    import oracle_lib as O
+    wl = workloads.random_network_workload(ncell=256)
+    n = wl.state.ncell
+    opt = dataclasses.replace(wl.opt, maximum_reaction_iterations=5, maximum_rreact_calls=3)
+    ref = wl.state.copy()
+    O.step_batch(wl.net, opt, ref, wl.dt, 0)
+    ok = ref["status"][0, :n] == 0
+    assert 20 < ok.sum() < n
+    exe = LU.build_host(wl.net, opt, str(tmp_path))
+    got = LU.run_host(exe, wl.state, wl.dt, f32=False, escalate=False)
+    np.testing.assert_array_equal(got["status"][0, :n] != 0, ~ok)
+    np.testing.assert_array_equal(got["num_sub_steps"][0, ok], ref["num_sub_steps"][0, ok])
+    for f in ("pri_molal", "total", "sec_molal", "mnrl_volfrac"):
+        a, b = ref[f][:, :n][:, ok], got[f][:, :n][:, ok]
+        assert np.max(np.abs(b - a) / np.maximum(np.abs(a), 1e-300)) <= CONC_TOL, f
+
+
 def test_prepare_refuses_other_networks():
     """the one-lane kernel takes aqueous networks of <= 16 primaries without
     the sandbox, per-cell active counts or rate prefactors (whose Jacobian

@@ -234,6 +234,46 @@ def test_rstep_cuts_then_fails_and_restores():
     np.testing.assert_array_equal(cc.get("total", 3), expect)
 
 
+def test_rreact_call_bound_truncates_the_sub_step_loop():
+    """options.maximum_rreact_calls stops RStep's sub-step loop after that
+    many RReact calls (reaction.F90:3721-3725 has no bound of its own): a
+    bound only truncates the loop, so under every bound a cell either fails
+    or ends bit-equal to the unbounded run, the failing cells shrink as the
+    bound grows, and a cell that took more sub-steps than the bound (one
+    RReact call each, at least) fails"""
+    wl = workloads.random_network_workload(ncell=512)
+    n = wl.state.ncell
+    opt = SolverOptions(**{**wl.opt.__dict__, "maximum_reaction_iterations": 5})
+    free = wl.state.copy()
+    O.step_batch(wl.net, opt, free, wl.dt, 0)
+    ok = free["status"][0, :n] == 0
+    nsub = free["num_sub_steps"][0, :n]
+    # both kinds present: cells done in one call and cells that cut dt
+    assert (ok & (nsub == 1)).sum() > 50 and (ok & (nsub > 2)).sum() > 10
+    fields = ("pri_molal", "total", "sec_molal", "mnrl_volfrac", "mnrl_rate", "num_iterations",
+              "num_sub_steps")
+    prev_fail = None
+    for bound in (1, 2, 3, 5, 8, 10000):
+        got = wl.state.copy()
+        O.step_batch(wl.net, SolverOptions(**{**opt.__dict__, "maximum_rreact_calls": bound}),
+                     got, wl.dt, 0)
+        st = got["status"][0, :n]
+        same = st == 0
+        assert np.all(st[~ok] != 0), bound
+        for f in fields:
+            np.testing.assert_array_equal(got[f][:, :n][:, same], free[f][:, :n][:, same],
+                                          err_msg="%s bound %d" % (f, bound))
+        assert np.all(st[ok & (nsub > bound)] != 0), bound
+        if bound == 1:
+            assert np.all(st[ok & (nsub == 1)] == 0)
+        fail = st != 0
+        if prev_fail is not None:
+            assert np.all(fail <= prev_fail), bound
+        prev_fail = fail
+    # a bound above every cell's call count is the unbounded run
+    assert np.array_equal(prev_fail, ~ok)
+
+
 def test_rstep_small_totals_clamped_and_restored():
     """totals <= 1e-40 (negative H+ totals here) are replaced by 1e-40 for
     the solve and restored after it (reaction.F90:3617-3631, 3727-3736)"""
