@@ -14,9 +14,12 @@ ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 # the same sources built -O3 with FMA contraction: timing of the CPU
 # baseline only, never the checker
 ORACLE_FAST_SO = os.path.join(ORACLE_DIR, "liboracle_fast.so")
+ORACLE_F90_SO = os.path.join(ORACLE_DIR, "liboracle_f90.so")
+ORACLE_F90_IEEE_SO = os.path.join(ORACLE_DIR, "liboracle_f90_ieee.so")
 
 _lib = None
 _fast = None
+_f90 = {}
 
 MAXCOMP, MAXCPLX, MAXMNRL, MAXRXN, SOMDEC_MAX_DOWN = 96, 512, 32, 32, 4
 
@@ -143,6 +146,37 @@ def fast_lib():
                                      C.c_double, C.c_int, C.c_void_p]
         _fast = L
     return _fast
+
+
+def f90_lib(ieee=False):
+    """the Fortran restatement of RStep (pfchem_oracle_rstep.F90): the -O3
+    build for the CPU-baseline timing, or (ieee) the build without FMA
+    contraction that the tests hold bit-equal to the checker"""
+    path = ORACLE_F90_IEEE_SO if ieee else ORACLE_F90_SO
+    if path not in _f90:
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_f90_step_batch.restype = C.c_int64
+        L.orc_f90_step_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                         C.c_double, C.c_int]
+        _f90[path] = L
+    return _f90[path]
+
+
+def step_batch_f90(net, opt, state, dt, nthreads=0, ieee=False):
+    """RStep over all cells of a host CellState (in place) on the Fortran
+    restatement.  Returns #failed; raises for networks outside its scope
+    (aqueous complexes and TST kinetic minerals, activity coefficients fixed
+    within the step, isothermal log K)."""
+    n = net.to_abi()
+    o = opt.to_abi()
+    s = state.to_abi()
+    r = f90_lib(ieee).orc_f90_step_batch(C.byref(n), C.byref(o), C.byref(s), state.ncell, dt,
+                                     nthreads)
+    if r < 0:
+        raise ValueError("network or options outside the Fortran restatement's scope")
+    return r
 
 
 def step_batch(net, opt, state, dt, nthreads=0, records=None, fast=False):
