@@ -445,12 +445,12 @@ def run_configs(args):
     res = {}
 
     def leg(key, wl, sample, variants, plain_steps=1, steps=3, extra=None, oracle=None,
-            band=False):
+            band=False, desc=None):
         t0 = time.time()
         n = wl.state.ncell
         ctx = Context(wl.net, wl.opt)
         dev0 = DeviceState(wl.state)
-        line = {"workload": wl.notes[:160], "cells": n, "dt_s": wl.dt}
+        line = {"workload": desc or wl.notes[:160], "cells": n, "dt_s": wl.dt}
         rng = np.random.default_rng(7)
         cells = np.sort(rng.choice(n, size=min(sample, n), replace=False))
         sub = wl.state.subset(cells)
@@ -468,9 +468,15 @@ def run_configs(args):
             t_cpu = time.perf_counter() - t1
             nfail = int((ref["status"][0] != 0).sum())
             build = "checker build, per sub-network"
+        it = ref["num_iterations"][0, :cells.size]
         line["cpu_oracle"] = {"cells_per_s": cells.size / t_cpu, "threads": nthr,
                               "sample_cells": int(cells.size), "failed_cells": int(nfail),
-                              "build": build}
+                              "build": build,
+                              "newton_iterations": {"mean": float(it.mean()),
+                                                    "p50": int(np.percentile(it, 50)),
+                                                    "p90": int(np.percentile(it, 90)),
+                                                    "p99": int(np.percentile(it, 99)),
+                                                    "max": int(it.max())}}
         okref = ref["status"][0] == 0
         spread = None
         if band:
@@ -541,6 +547,15 @@ def run_configs(args):
     allv = [abi.KERNEL_PLAIN_FP64, abi.KERNEL_BATCHED_FP64, abi.KERNEL_BATCHED_MIXED]
     # C1: the calcite system, 10x10x10 cells
     leg("C1_calcite_1000", workloads.calcite_workload(1000), 1000, allv, plain_steps=3, steps=5)
+    # C2 with uneven Newton counts: the headline network, each cell's
+    # totals drawn +-0.5 decades around the reference composition and mixed
+    # with its neighbour's at a weight U[0, 1] (the headline draws +-0.25
+    # decades and U[0, 0.3])
+    leg("C2u_uneven_random15_1048576",
+        workloads.random_network_workload(ncell=1 << 20, spread=0.5, wmax=1.0), 32768, allv,
+        desc="the headline's random 15-primary network; initial free molalities "
+             "10^(xref + U[-0.5, 0.5]) per species, totals mixed with the next cell's at "
+             "w ~ U[0, 1], ionic strength logU[1e-3, 0.5]")
 
     # C4: adjoint of the step on the 15-primary network (65 parameters:
     # 5 log10 k + 60 log10 K), 262,144 cells; objective: sum of end-of-step

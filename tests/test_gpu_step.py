@@ -71,6 +71,26 @@ def test_random_network_parity(randnet, variant):
     _check(wl, ref, rec, got, "random15/" + abi.KERNEL_NAMES[variant])
 
 
+@pytest.fixture(scope="module")
+def uneven():
+    # the same network, initial states drawn +-0.5 decades around the
+    # reference composition and mixed with the next cell's at w ~ U[0, 1]:
+    # 1 to ~40 Newton iterations per cell
+    wl = workloads.random_network_workload(ncell=16384, spread=0.5, wmax=1.0)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    it = ref["num_iterations"][0, :wl.state.ncell]
+    assert it.max() >= 3 * np.median(it)
+    return wl, ref, rec
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_uneven_iteration_counts_parity(uneven, variant):
+    wl, ref, rec = uneven
+    got, _ = G.run_gpu(wl, variant)
+    _check(wl, ref, rec, got, "random15-uneven/" + abi.KERNEL_NAMES[variant])
+
+
 def _run_shape(wl, variant, lanes, rows, block):
     """run with the requested team shape, halving the workgroup size until
     the team tiles fit the device's LDS; None when no size fits"""
