@@ -142,6 +142,10 @@ void* static_scratch_ptr(int device) {
 // stream the earlier launch used.
 const int kStateRing = 256;
 constexpr size_t kCursorWords = 1024;  // 8 KiB of cursors per descriptor slot
+// handed-over cells above which the FP64 one-lane kernel runs before the
+// team pass (the team pass alone takes a short list faster: its time is one
+// cell's latency)
+constexpr int kLaneListMin = 4096;
 
 unsigned long long* slot_cursors(pfchem_ctx* ctx, int k) {
   return ctx->counter + (size_t)k * kCursorWords;
@@ -967,6 +971,20 @@ static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double d
   if (listed) {
     e = launch_escalated_list(st, ncell, ctx->esc_list, ctx->esc_list + ncell, s);
     if (e != hipSuccess) return hip_fail(ctx, e, "escalated-cell list");
+    if (mixed) {
+      // a long list goes through the FP64 one-lane kernel first (per cell
+      // several times cheaper than a team, at a longer latency for its
+      // slowest cell); the team pass takes what that pass hands over.  A
+      // short list (the kernel reads its length) is left to the team pass.
+      LaneArgs b = a;
+      b.rescue = 1;
+      b.list = ctx->esc_list;
+      b.list_count = ctx->esc_list + ncell;
+      b.list_min = kLaneListMin;
+      e = lane_launch(ctx->lane_src, ctx->device, 0, b, ctx->num_cus,
+                      ctx->launch[PFCHEM_KERNEL_BATCHED_FP64].blocks_per_cu, cursors, s);
+      if (e != hipSuccess) return hip_fail(ctx, e, "lane kernel launch (FP64 list)");
+    }
   }
   e = launch_team_step(ctx->dnet, ctx->dnet_dev, slot, ncell, dt, kTeamRescue64,
                        ctx->team_default, cursors, ctx->num_cus, ctx->nxcd, s, true);

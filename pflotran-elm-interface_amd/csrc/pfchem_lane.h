@@ -44,9 +44,11 @@ struct LaneArgs {
   int nxcd;
   int rescue;
   int escalate;
-  int pad;
+  int list_min;
+  const int* list;
+  const int* list_count;
 };
-static_assert(sizeof(LaneArgs) == 192, "LaneArgs layout");
+static_assert(sizeof(LaneArgs) == 208, "LaneArgs layout");
 
 // adjoint kernel arguments; the same layout as pflane::SensArgs
 struct LaneSensArgs {

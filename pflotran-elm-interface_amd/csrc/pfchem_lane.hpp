@@ -75,8 +75,11 @@ struct Args {
   int nxcd;
   int rescue;    // 1: only cells whose status is kEscalate
   int escalate;  // 1: hand failing cells to a later pass (status kEscalate)
-  int pad;
+  int list_min;  // with a list: the pass does nothing unless it holds more cells
+  const int* list;        // NULL, or the cells to step: list[0 .. *list_count)
+  const int* list_count;  // device memory, written by an earlier kernel
 };
+static_assert(sizeof(Args) == 208, "Args layout");
 
 template <int V>
 struct IC {
@@ -213,7 +216,12 @@ __device__ __forceinline__ void lane_step(const Args& A) {
   const int lane = (int)__lane_id();
   const int nxcd = A.nxcd;
   const int xcd = (int)(blockIdx.x % (unsigned)nxcd);
-  const i64 chunk = (A.ncell + nxcd - 1) / nxcd;
+  // cells of this pass: all, or a list an earlier kernel wrote (a short
+  // list is left to a later pass: the grid returns at once)
+  const bool listed = A.list != nullptr;
+  const i64 nwork = listed ? (i64)*A.list_count : A.ncell;
+  if (listed && nwork <= (i64)A.list_min) return;
+  const i64 chunk = (nwork + nxcd - 1) / nxcd;
 
   // wave-uniform queue of claimed cells [qn, qe); XCD ranges found empty
   i64 qn = 0, qe = 0;
@@ -260,7 +268,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
           }
           const int xr = xcd + steal < nxcd ? xcd + steal : xcd + steal - nxcd;
           const i64 lo = (i64)xr * chunk;
-          const i64 hi = lo + chunk < A.ncell ? lo + chunk : A.ncell;
+          const i64 hi = lo + chunk < nwork ? lo + chunk : nwork;
           const int first = __builtin_ctzll(__ballot(1));
           unsigned long long base = 0;
           if (lane == first) base = atomicAdd(A.cursors + 16 * xr, (unsigned long long)kClaim);
@@ -276,7 +284,7 @@ __device__ __forceinline__ void lane_step(const Args& A) {
         const i64 avail = qe - qn;
         const int cnt = __popcll(mask);
         if (needy && rk < avail) {
-          cell = qn + rk;
+          cell = listed ? (i64)A.list[qn + rk] : qn + rk;
           needy = false;
           // a rescue pass takes only the cells an earlier pass handed over
           if (A.rescue && A.status[cell] != kEscalate) {

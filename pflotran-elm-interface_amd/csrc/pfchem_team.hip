@@ -99,21 +99,40 @@ bucket_cells(const int32_t* nact, int64_t ncell, int naq, int32_t* lists, int32_
 __global__ void clear_count(int32_t* count) {
   if (threadIdx.x < 1) count[threadIdx.x] = 0;
 }
-__global__ void __launch_bounds__(256)
+// one atomic per workgroup of kListBlock threads: its handed-over cells
+// take consecutive entries, in cell order within the workgroup
+constexpr int kListBlock = 1024;
+__global__ void __launch_bounds__(kListBlock)
 escalated_cells(const int32_t* status, int64_t ncell, int32_t* list, int32_t* count) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell || status[c] != kTeamEscalate) return;
-  list[atomicAdd(count, 1)] = (int32_t)c;
+  __shared__ int wave_off[kListBlock / 64];
+  __shared__ int block_base;
+  const int64_t c = (int64_t)blockIdx.x * kListBlock + threadIdx.x;
+  const bool esc = c < ncell && status[c] == kTeamEscalate;
+  const unsigned long long mask = __ballot(esc);
+  const int lane = (int)__lane_id();
+  const int wave = (int)(threadIdx.x >> 6);
+  if (lane == 0) wave_off[wave] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int w = 0; w < kListBlock / 64; ++w) {
+      const int k = wave_off[w];
+      wave_off[w] = total;
+      total += k;
+    }
+    block_base = total > 0 ? atomicAdd(count, total) : 0;
+  }
+  __syncthreads();
+  if (esc) list[block_base + wave_off[wave] + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)c;
 }
 }  // namespace
 
 hipError_t launch_escalated_list(const int32_t* status, int64_t ncell, int32_t* list,
                                  int32_t* count, hipStream_t stream) {
   clear_count<<<1, 64, 0, stream>>>(count);
-  const int block = 256;
   if (ncell > 0)
-    escalated_cells<<<dim3((unsigned)((ncell + block - 1) / block)), dim3(block), 0, stream>>>(
-        status, ncell, list, count);
+    escalated_cells<<<dim3((unsigned)((ncell + kListBlock - 1) / kListBlock)), dim3(kListBlock), 0,
+                      stream>>>(status, ncell, list, count);
   return hipGetLastError();
 }
 

@@ -54,9 +54,20 @@ int main(int argc, char** argv) {
   a.rescue = 0;
   if (f32) {
     pfchem_lane_step_f32(a);
-    if (a.escalate && !std::getenv("PFCHEM_HOST_NO_RESCUE")) {
+    if (a.escalate && !buf[17].empty() && !std::getenv("PFCHEM_HOST_NO_RESCUE")) {
+      // the FP64 pass over the list of handed-over cells, as the library
+      // runs it after the FP32 pass
+      std::vector<int> list;
+      const int* st = (const int*)buf[17].data();
+      for (long long c = 0; c < a.ncell; ++c)
+        if (st[c] == 16) list.push_back((int)c);
+      const int count = (int)list.size();
+      list.push_back(0);  // a valid address when the list is empty
       cursors[0] = 0;
       a.rescue = 1;
+      a.list = list.data();
+      a.list_count = &count;
+      a.list_min = 0;
       pfchem_lane_step_f64(a);
     }
   } else {
