@@ -55,6 +55,9 @@ def main():
                                        "transport", "equilibrate"],
                     help="time only this kernel of the step, or only the OSRT transport "
                          "leg (profiling runs), and exit")
+    ap.add_argument("--uneven", action="store_true",
+                    help="with --only: the uneven-iteration draw of the same network "
+                         "(states +-0.5 decades, neighbour mixing w ~ U[0, 1])")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default launch shapes (skip pfchem_autotune)")
     ap.add_argument("--tune-cache", default=None,
@@ -88,7 +91,12 @@ def main():
     from pfchem.lib import Context, DeviceState
 
     t0 = time.time()
-    wl = workloads.random_network_workload(ncell=args.cells)
+    if args.uneven and not args.only:
+        print("--uneven applies to the --only runs (the default run reports it as a "
+              "config line)", file=sys.stderr)
+        return 2
+    wl = (workloads.random_network_workload(ncell=args.cells, spread=0.5, wmax=1.0)
+          if args.uneven else workloads.random_network_workload(ncell=args.cells))
     net, opt, host = wl.net, wl.opt, wl.state
     n = host.ncell
     log("[bench] workload %s: %d cells generated in %.1f s" % (wl.name, n, time.time() - t0))
