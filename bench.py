@@ -617,7 +617,7 @@ def run_configs(args):
         return V.run_oracle_variable(dataclasses.replace(wl, state=sub), nthreads=nthr)[0]
 
     leg("C5_variable_262144", workloads.variable_network_workload(ncell=262144), 2048, allv,
-        plain_steps=1, steps=2, oracle=variable_oracle)
+        plain_steps=1, steps=2, oracle=variable_oracle, tune_cells=1 << 16)
     # C3: SomDec decomposition, 500,000 columns x 15 layers
     leg("C3_somdec_7500000", workloads.somdec_workload(), 32768, allv, steps=2, band=True,
         tune_cells=1 << 20)
