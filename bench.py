@@ -28,7 +28,8 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path[:0] = [os.path.join(ROOT, "pflotran-elm-interface_amd"), os.path.join(ROOT, "oracle")]
+sys.path[:0] = [os.path.join(ROOT, "pflotran-elm-interface_amd"), os.path.join(ROOT, "oracle"),
+                os.path.join(ROOT, "tests")]
 
 METRIC = ("reaction-step cells/s @ rel.resid≤1e-10: plain FP64 vs batched "
           "mixed+adjoint")
@@ -234,6 +235,7 @@ def main():
     iters_ref = iters_gpu
     if not args.no_cpu:
         import oracle_lib as O
+        import gpu_util as G
         # the host share of one GPU: OMP_NUM_THREADS when set, else <= 16 cores
         nthr = (args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                 or min(16, os.cpu_count() or 1))
@@ -289,7 +291,15 @@ def main():
                 # affinity
                 floor = 0.0
                 if f == "total":
+                    # relative to the magnitude of the terms a total is summed
+                    # from (a cancelling total amplifies its terms' rounding);
+                    # the error relative to the total itself is reported too
                     floor = 1e-12 * np.abs(a).max(axis=1, keepdims=True)
+                    d0 = np.abs(g[f][:, :n][:, ok] - a)
+                    with np.errstate(divide="ignore", invalid="ignore"):
+                        errs["total_rel_to_itself"] = float(np.where(
+                            d0 == 0, 0.0, d0 / np.maximum(np.abs(a), floor)).max())
+                    floor = np.maximum(floor, G.total_term_scale(net, ref, n)[:, ok])
                 elif f == "mnrl_rate":
                     ka = net.kinmnrl_rate_constant[:, None] * host["mnrl_area"][:, :n][:, ok]
                     floor = np.maximum(ka, 1e-300)
@@ -421,7 +431,11 @@ def _timed_steps(ctx, dev0, dt, variant, nsteps, nwarm):
     return out, total / nsteps
 
 
-def _conc_err(ref, got, cells, fields=("pri_molal", "sec_molal", "total", "immobile")):
+def _conc_err(ref, got, cells, fields=("pri_molal", "sec_molal", "total", "immobile"),
+              tscale=None):
+    """max relative error per field over the given cells; with tscale (the
+    magnitude of each total's terms) totals are measured against it and
+    their error relative to themselves is reported beside"""
     import numpy as np
     worst = {}
     for f in fields:
@@ -431,6 +445,11 @@ def _conc_err(ref, got, cells, fields=("pri_molal", "sec_molal", "total", "immob
             continue
         floor = 1e-12 * np.abs(a).max(axis=1, keepdims=True) if f == "total" else 0.0
         d = np.abs(b - a)
+        if f == "total" and tscale is not None:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                r0 = np.where(d == 0, 0.0, d / np.maximum(np.abs(a), floor))
+            worst["total_rel_to_itself"] = float(r0.max())
+            floor = np.maximum(floor, tscale[:, cells])
         with np.errstate(divide="ignore", invalid="ignore"):
             r = np.where(d == 0, 0.0, d / np.maximum(np.abs(a), floor))
         worst[f] = float(r.max())
@@ -493,6 +512,10 @@ def run_configs(args):
                                                     "p99": int(np.percentile(it, 99)),
                                                     "max": int(it.max())}}
         okref = ref["status"][0] == 0
+        import gpu_util as G
+        tscale = (G.total_term_scale(wl.net, ref, cells.size)
+                  if not wl.net.nimmobile and wl.net.somdec is None and not wl.opt.variable_ncomp
+                  else None)
         spread = None
         if band:
             # the oracle's own answers under a ten-fold looser and tighter
@@ -530,9 +553,10 @@ def run_configs(args):
             line[name] = {"ms_per_step": ms, "cells_per_s": n / (ms * 1e-3),
                           "failed_cells": int((got["status"][0, :n] != 0).sum()),
                           "launch": ctx.get_launch(v),
-                          "max_rel_conc_err": _conc_err(ref, g, np.where(same)[0]),
+                          "max_rel_conc_err": _conc_err(ref, g, np.where(same)[0], tscale=tscale),
                           "other_path_cells": int((ok & ~same).sum()),
-                          "max_rel_conc_err_all_cells": _conc_err(ref, g, np.where(ok)[0])}
+                          "max_rel_conc_err_all_cells": _conc_err(ref, g, np.where(ok)[0],
+                                                                  tscale=tscale)}
             if spread is not None:
                 worst, outside = 0.0, np.zeros(cells.size, dtype=bool)
                 for f, bd in spread.items():

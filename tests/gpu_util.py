@@ -52,10 +52,30 @@ def rate_scale(net, state, ncell):
     return (k * a)[:, None]
 
 
+def total_term_scale(net, state, ncell):
+    """per component and cell: the magnitude of the terms its total is summed
+    from, (|m_i| + sum_k |nu_ik| m_k) rho / 1000 (RTotal, reaction.F90:4958-
+    5066).  A total is a signed sum; where it cancels, its relative error is
+    the terms' error times |terms| / |total|, so its error is measured
+    against this scale."""
+    naq = net.naqcomp
+    m = np.abs(state["pri_molal"][:naq, :ncell])
+    scale = m.copy()
+    if len(net.eqcplx_species):
+        nu = np.zeros((naq, len(net.eqcplx_species)))
+        for k, (sp, st) in enumerate(zip(net.eqcplx_species, net.eqcplx_stoich)):
+            for s_, v in zip(sp, st):
+                nu[s_, k] += abs(v)
+        scale += nu @ np.abs(state["sec_molal"][:len(net.eqcplx_species), :ncell])
+    return scale * state["den_kg"][0, :ncell][None, :] * 1.0e-3
+
+
 def compare(ref, got, ncell, fields=FIELDS_F64, floor_frac=1e-12, net=None):
     """max relative error per field over the first ncell cells.  Totals can be
     near zero, so their relative error uses a floor of floor_frac times the
-    largest magnitude of that species over the cells.  Mineral rates
+    largest magnitude of that species over the cells; with the network given,
+    a total's error is measured against the magnitude of its terms
+    (total_term_scale).  Mineral rates
     Im = -A k (1 - QK) vanish at equilibrium, where their relative error is
     the concentration error amplified by QK / |1 - QK|: with the network
     given, they are compared relative to max(|Im|, k A) (k A: the rate at
@@ -69,6 +89,8 @@ def compare(ref, got, ncell, fields=FIELDS_F64, floor_frac=1e-12, net=None):
         floor = 0.0
         if f == "total" or (f == "mnrl_rate" and net is None):
             floor = floor_frac * np.abs(a).max(axis=1, keepdims=True)
+            if f == "total" and net is not None and not net.nimmobile and net.somdec is None:
+                floor = np.maximum(floor, total_term_scale(net, ref, ncell))
         elif f == "mnrl_rate":
             floor = rate_scale(net, ref, ncell)
         out[f] = float(rel_err(b, a, floor).max())

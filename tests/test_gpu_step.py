@@ -2,8 +2,14 @@
 FP64 CPU oracle on the same seeded inputs.
 
 Bar (stated per test): relative concentration error <= 1e-8 against the
-oracle, and the oracle's FP64 relative residual of the kernel's answer
-<= 1e-10.  Calls go through the C ABI (libpfchem.so via ctypes).
+oracle (free and secondary molalities relative to themselves; totals
+relative to the magnitude of the terms they are summed from, since a total
+that cancels amplifies its terms' rounding: on the uneven draw a total of
+1.1e-9 mol/L summed from terms near 1e-4 differs from the oracle's by
+1.2e-9 of itself in the plain FP64 kernel and 4e-7 in the mixed one, whose
+free molalities agree to 5e-12), and the oracle's FP64 relative residual of
+the kernel's answer <= 1e-10.  Calls go through the C ABI (libpfchem.so via
+ctypes).
 """
 import numpy as np
 import pytest
@@ -42,7 +48,8 @@ def _check(wl, ref, rec, got, label):
     print("case", label, flush=True)
     assert np.all(got["status"][0, :n] == 0), "%s: failed cells" % label
     errs = G.compare(ref, got, n, net=wl.net)
-    print(label, errs, flush=True)
+    plain = G.compare(ref, got, n, fields=("total",))
+    print(label, errs, "totals relative to themselves:", plain, flush=True)
     for f, e in errs.items():
         assert e <= CONC_TOL, "%s: %s rel err %.3e > %.1e" % (label, f, e, CONC_TOL)
     rr, bar = G.gate_rel_residual(wl, got, rec, range(n), ref)
@@ -89,6 +96,17 @@ def test_uneven_iteration_counts_parity(uneven, variant):
     wl, ref, rec = uneven
     got, _ = G.run_gpu(wl, variant)
     _check(wl, ref, rec, got, "random15-uneven/" + abi.KERNEL_NAMES[variant])
+
+
+def test_long_hand_over_list():
+    """the mixed step on 131,072 cells of the uneven draw: the FP32 pass hands
+    over about a tenth of the cells, more than the 4,096 above which the
+    FP64 one-lane kernel steps the listed cells before the team pass"""
+    wl = workloads.random_network_workload(ncell=131072, spread=0.5, wmax=1.0)
+    ref, rec, nfail = G.run_oracle(wl)
+    assert nfail == 0
+    got, _ = G.run_gpu(wl, abi.KERNEL_BATCHED_MIXED)
+    _check(wl, ref, rec, got, "random15-uneven-131072/batched_mixed")
 
 
 def _run_shape(wl, variant, lanes, rows, block):
