@@ -12,6 +12,10 @@ Invariants: every index of [0, nwork) is handed out exactly once, a team
 holds at most one cell, every team ends exhausted, and no team is told the
 work is gone while a range still holds cells.  The shapes with 64 and 8
 teams per wavefront (1xR and 8x1) are covered as well as the shipped ones.
+
+The one-lane kernel's hand-out (csrc/pfchem_lane.hpp) is modelled the same
+way, with its cell-list mode and the rescue filter: every index claimed once,
+every listed handed-over cell stepped once, the others skipped.
 """
 import numpy as np
 import pytest
@@ -146,3 +150,103 @@ def test_the_model_catches_a_planted_queue_slip():
     k = Launch(4096, 8, 8, 64, 16, False, rng, slip=1)
     _run(k, rng)
     assert not np.all(k.handed == 1)
+
+
+# ---------------------------------------------------------------------------
+# the one-lane kernel's hand-out (csrc/pfchem_lane.hpp, "work stealing: hand
+# out cells"): one cell per lane, 64-index claims, an optional cell list and
+# the rescue filter (a pass over handed-over cells skips the others)
+# ---------------------------------------------------------------------------
+LANE_CLAIM = 64  # kClaim
+
+
+class LaneLaunch:
+    def __init__(self, nwork, nxcd, waves, listed, rescue, rng):
+        self.nwork, self.nxcd = nwork, nxcd
+        self.cursors = [0] * nxcd
+        self.chunk = (nwork + nxcd - 1) // nxcd
+        self.list = rng.permutation(4 * max(nwork, 1))[:nwork] if listed else None
+        ncell = 4 * max(nwork, 1) if listed else nwork
+        self.escalated = rng.random(ncell) < 0.5 if rescue else np.ones(ncell, dtype=bool)
+        self.rescue = rescue
+        self.claimed = np.zeros(nwork, dtype=np.int64)  # per list / range index
+        self.stepped = np.zeros(ncell, dtype=np.int64)  # per cell
+        self.waves = [LaneWave(self, w % nxcd, rng) for w in range(waves)]
+
+
+class LaneWave:
+    def __init__(self, launch, xcd, rng):
+        self.k, self.xcd, self.rng = launch, xcd, rng
+        self.cell = [-1] * 64
+        self.busy = [0] * 64
+        self.exhausted = [False] * 64
+        self.qn = self.qe = 0
+        self.steal = 0
+
+    def done(self):
+        return all(c < 0 for c in self.cell) and all(self.exhausted)
+
+    def round(self):
+        k = self.k
+        needy = [self.cell[l] < 0 and not self.exhausted[l] for l in range(64)]
+        mask = list(needy)
+        while any(mask):
+            if self.qn >= self.qe:
+                if self.steal >= k.nxcd:
+                    for l in range(64):
+                        if needy[l]:
+                            self.exhausted[l] = True
+                        needy[l] = False
+                    break
+                xr = self.xcd + self.steal if self.xcd + self.steal < k.nxcd else \
+                    self.xcd + self.steal - k.nxcd
+                lo = xr * k.chunk
+                hi = min(lo + k.chunk, k.nwork)
+                base = k.cursors[xr]          # atomicAdd by the first active lane
+                k.cursors[xr] += LANE_CLAIM
+                if lo + base >= hi:
+                    self.steal += 1
+                    continue
+                self.qn = lo + base
+                self.qe = min(self.qn + LANE_CLAIM, hi)
+            avail = self.qe - self.qn
+            cnt = sum(mask)
+            for l in range(64):
+                rk = sum(mask[:l])
+                if needy[l] and rk < avail:
+                    idx = self.qn + rk
+                    k.claimed[idx] += 1
+                    c = int(k.list[idx]) if k.list is not None else idx
+                    self.cell[l] = c
+                    needy[l] = False
+                    if k.rescue and not k.escalated[c]:
+                        self.cell[l] = -1
+                        needy[l] = True
+                    else:
+                        k.stepped[c] += 1
+                        self.busy[l] = int(self.rng.integers(1, 8))
+            self.qn += min(cnt, avail)
+            mask = list(needy)
+        for l in range(64):
+            if self.cell[l] >= 0:
+                self.busy[l] -= 1
+                if self.busy[l] == 0:
+                    self.cell[l] = -1
+
+
+@pytest.mark.parametrize("listed,rescue", [(False, False), (False, True), (True, True), (True, False)])
+@pytest.mark.parametrize("nwork,waves", [(1, 8), (1000, 8), (20000, 16), (0, 8), (4099, 24)])
+def test_lane_hand_out(listed, rescue, nwork, waves):
+    rng = np.random.default_rng(nwork * 7 + waves + 2 * listed + rescue)
+    k = LaneLaunch(nwork, 8, waves, listed, rescue, rng)
+    for _ in range(500000):
+        live = [w for w in k.waves if not w.done()]
+        if not live:
+            break
+        live[int(rng.integers(len(live)))].round()
+    assert all(w.done() for w in k.waves)
+    assert np.all(k.claimed == 1)
+    want = np.zeros_like(k.stepped)
+    cells = k.list if listed else np.arange(nwork)
+    want[cells] = k.escalated[cells]
+    np.testing.assert_array_equal(k.stepped, want)
