@@ -341,7 +341,11 @@ const char* pfchem_last_error(const pfchem_ctx* ctx);
    ncell cells whose state pointers are DEVICE pointers.  Launches on
    `stream` (a hipStream_t, NULL = default stream) and returns without
    synchronizing.  Per-cell outcome goes to state->status; the call itself
-   returns PFCHEM_OK if the launch succeeded.
+   returns PFCHEM_OK if the launch succeeded.  Calls on one context may use
+   different streams; steps that use the context's scratch lists (the
+   one-lane kernel's hand-over list, the status array kept when
+   state->status is NULL, the size buckets) run one after another on the
+   device, whatever their streams.
    Replaces RStep (reaction.F90:3564) applied to every local cell. */
 int pfchem_step(pfchem_ctx* ctx, int64_t ncell, const pfchem_state* state,
                 double dt, int variant, void* stream);
