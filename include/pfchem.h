@@ -437,6 +437,16 @@ int pfchem_get_launch(const pfchem_ctx* ctx, int variant, pfchem_launch* cfg);
    (optional): text table of every candidate's time, difference and
    outcome.  Allocates device scratch for a copy of the state's output
    fields. */
+/* Measurement hook (no counterpart in the reference; for benchmarks and
+   profilers that time the dominant kernel on its own stream).  With timing
+   on, every batched pfchem_step that runs the one-lane kernel records a
+   pair of HIP events on its stream around that kernel's pass over every
+   cell (the per-launch cursor clear included).  pfchem_pass_time waits
+   for the last recorded pair and returns the pass's duration in ms;
+   PFCHEM_ERR_INVALID when no step recorded one since timing was turned on. */
+int pfchem_set_pass_timing(pfchem_ctx* ctx, int on);
+int pfchem_pass_time(pfchem_ctx* ctx, float* ms);
+
 #define PFCHEM_TUNE_FORCE 1
 #define PFCHEM_TUNE_QUICK 2
 int pfchem_autotune(pfchem_ctx* ctx, int variant, int64_t ncell, const pfchem_state* state,

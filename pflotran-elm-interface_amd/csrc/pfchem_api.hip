@@ -47,6 +47,10 @@ struct pfchem_ctx {
   // event after its last launch and the next one's stream waits for it
   hipEvent_t scratch_done = nullptr;
   bool scratch_done_used = false;
+  // pfchem_set_pass_timing: events around the one-lane pass of a step
+  bool pass_timing = false;
+  bool pass_recorded = false;
+  hipEvent_t pass_t0 = nullptr, pass_t1 = nullptr;
   pfchem_network net{};   // sizes only (host pointers are not kept)
   pfchem_options opt{};
   bool has_somdec = false;
@@ -838,6 +842,8 @@ int pfchem_destroy(pfchem_ctx* ctx) {
   for (int k = 0; k < kStateRing; ++k)
     if (ctx->ring_done[k]) (void)hipEventDestroy(ctx->ring_done[k]);
   if (ctx->scratch_done) (void)hipEventDestroy(ctx->scratch_done);
+  if (ctx->pass_t0) (void)hipEventDestroy(ctx->pass_t0);
+  if (ctx->pass_t1) (void)hipEventDestroy(ctx->pass_t1);
   delete ctx;
   return PFCHEM_OK;
 }
@@ -964,8 +970,13 @@ static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double d
   unsigned long long* cursors = slot_cursors(ctx, k);
   // a step on another stream may still use the list / status scratch
   e = scratch_order(ctx, s);
+  if (e == hipSuccess && ctx->pass_timing) e = hipEventRecord(ctx->pass_t0, s);
   if (e == hipSuccess)
     e = lane_launch(ctx->lane_src, ctx->device, mixed ? 1 : 0, a, ctx->num_cus, bpc, cursors, s);
+  if (e == hipSuccess && ctx->pass_timing) {
+    e = hipEventRecord(ctx->pass_t1, s);
+    if (e == hipSuccess) ctx->pass_recorded = true;
+  }
   if (e != hipSuccess)
     return hip_fail(ctx, e, mixed ? "lane kernel launch (FP32)" : "lane kernel launch (FP64)");
   if (listed) {
@@ -991,6 +1002,26 @@ static int lane_step(pfchem_ctx* ctx, int64_t ncell, const DevState& d, double d
   if (e == hipSuccess) e = release_slot(ctx, k, s);
   if (e == hipSuccess) e = scratch_release(ctx, s);
   if (e != hipSuccess) return hip_fail(ctx, e, "team rescue launch");
+  return PFCHEM_OK;
+}
+
+int pfchem_set_pass_timing(pfchem_ctx* ctx, int on) {
+  if (!ctx) return PFCHEM_ERR_INVALID;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess && on && !ctx->pass_t0) e = hipEventCreate(&ctx->pass_t0);
+  if (e == hipSuccess && on && !ctx->pass_t1) e = hipEventCreate(&ctx->pass_t1);
+  if (e != hipSuccess) return hip_fail(ctx, e, "pass timing events");
+  ctx->pass_timing = on != 0;
+  ctx->pass_recorded = false;
+  return PFCHEM_OK;
+}
+
+int pfchem_pass_time(pfchem_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return PFCHEM_ERR_INVALID;
+  if (!ctx->pass_recorded) return fail(ctx, PFCHEM_ERR_INVALID, "no timed one-lane pass recorded");
+  hipError_t e = hipEventSynchronize(ctx->pass_t1);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms, ctx->pass_t0, ctx->pass_t1);
+  if (e != hipSuccess) return hip_fail(ctx, e, "pass timing");
   return PFCHEM_OK;
 }
 

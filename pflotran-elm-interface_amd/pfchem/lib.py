@@ -45,6 +45,8 @@ def load(path: str = LIB_PATH):
     L.pfchem_last_error.restype = C.c_char_p
     L.pfchem_last_error.argtypes = [vp]
     L.pfchem_step.argtypes = [vp, C.c_int64, vp, C.c_double, C.c_int, vp]
+    L.pfchem_set_pass_timing.argtypes = [vp, C.c_int]
+    L.pfchem_pass_time.argtypes = [vp, C.POINTER(C.c_float)]
     L.pfchem_step_host.argtypes = [vp, C.c_int64, vp, C.c_double, C.c_int]
     L.pfchem_update_activity.argtypes = [vp, C.c_int64, vp, vp]
     L.pfchem_residual.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp]
@@ -192,6 +194,16 @@ class Context:
             sp = torch.cuda.current_stream().cuda_stream
         n = st.ncell if ncell is None else ncell
         self._check(self.L.pfchem_step(self.h, n, C.byref(s), dt, variant, sp), "pfchem_step")
+
+    def set_pass_timing(self, on: bool = True):
+        """pfchem_set_pass_timing: HIP events around the one-lane pass"""
+        self._check(self.L.pfchem_set_pass_timing(self.h, 1 if on else 0), "pfchem_set_pass_timing")
+
+    def pass_time(self) -> float:
+        """pfchem_pass_time: the last timed one-lane pass, in ms"""
+        ms = C.c_float()
+        self._check(self.L.pfchem_pass_time(self.h, C.byref(ms)), "pfchem_pass_time")
+        return float(ms.value)
 
     def step_host(self, st: CellState, dt: float, variant: int = abi.KERNEL_BATCHED_MIXED):
         s = st.to_abi()

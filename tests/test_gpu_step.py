@@ -192,3 +192,21 @@ def test_ld_padding_and_partial_range(randnet):
     errs = G.compare(ref, got, half, net=wl.net)
     assert max(errs.values()) <= CONC_TOL
     assert np.array_equal(got["pri_molal"][:, half:], padded.state["pri_molal"][:, half:])
+
+
+def test_pass_timing_hook(randnet):
+    """pfchem_set_pass_timing / pfchem_pass_time: no time before a timed
+    step; after one, the one-lane pass's duration, and the step's answer is
+    the untimed one's"""
+    from pfchem.lib import Context, DeviceState, PfchemError
+    wl, ref, rec = randnet
+    ctx = Context(wl.net, wl.opt)
+    ctx.set_pass_timing(True)
+    with pytest.raises(PfchemError):
+        ctx.pass_time()
+    got, _ = G.run_gpu(wl, abi.KERNEL_BATCHED_MIXED, ctx=ctx)
+    ms = ctx.pass_time()
+    print("one-lane pass %.3f ms" % ms, flush=True)
+    assert 0.0 < ms < 1.0e4
+    _check(wl, ref, rec, got, "random15/timed-pass")
+    ctx.set_pass_timing(False)
